@@ -1,0 +1,183 @@
+"""Benchmark: Jaccard coefficients/s on one MI355X.
+
+Default workload (the configuration the headline metric is quoted on):
+RMAT scale 22, edge factor 16, a/b/c = .57/.19/.19, symmetrised, deduped,
+sorted int32 CSR (~1.28e8 stored edges), float32 edge weights uniform in
+(0,1]. One step = vertex weights from edge weights (the reference driver's
+weighted preamble) + the edge-centric weighted Jaccard of every stored edge.
+Inputs are resident in HBM before the timed region.
+
+Prints one JSON line. ``roofline`` uses the algorithmic bytes
+  B = 4*sum_{stored (u,v)} min(d_u,d_v) + 4(n+1) + 4m + 4m  (+ 4m + 8n weighted)
+over the dominant kernel's average launch time (HIP events on its stream).
+``cpu_baseline`` times the CPU restatement of the reference's edge-centric
+loop (binary search per shorter-row element) on a bounded random sample of
+the same graph's edges, with all host threads.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "sycl-jaccard_amd"))
+
+METRIC = "Jaccard coeffs/s + % HBM roofline; batched graphs/s & p99 latency vs plain"
+HBM_PEAK = 8.0e12
+
+WORKLOADS = {
+    # name: (generator kwargs, weighted, description)
+    "rmat22_weighted": (dict(kind="rmat", scale=22, edge_factor=16, seed=1), True,
+                        "RMAT-22 ef16 weighted (0,1]"),
+    "rmat16": (dict(kind="rmat", scale=16, edge_factor=16, seed=1), False, "RMAT-16 ef16 unweighted"),
+    "chunglu20": (dict(kind="chunglu", n=1 << 20, seed=1), False,
+                  "Chung-Lu 2^20 gamma 2.1 mean 32 max 2^17 unweighted"),
+}
+
+
+def make_graph(spec, weighted):
+    from jaccard_amd import graphs
+    kind = spec["kind"]
+    if kind == "rmat":
+        return graphs.rmat(spec["scale"], spec["edge_factor"], seed=spec["seed"], weighted=weighted)
+    return graphs.chung_lu(n=spec["n"], seed=spec["seed"], weighted=weighted)
+
+
+def algorithmic_bytes(g, st, weighted):
+    b = 4 * st["sum_min_deg"] + 4 * (g.n + 1) + 4 * g.m + 4 * g.m
+    if weighted:
+        b += 4 * g.m + 8 * g.n
+    return int(b)
+
+
+def cpu_baseline(g, budget_s=15.0, seed=0):
+    """Reference EC loop (oracle restatement) on a random edge sample, all host threads."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    threads = oracle.max_threads()
+    rng = np.random.default_rng(seed)
+    n_s = 2000
+    t_used = 0.0
+    done = 0
+    while True:
+        edges = rng.integers(0, g.m, n_s)
+        t0 = time.perf_counter()
+        oracle.ec_edges(g, edges, threads)
+        dt = time.perf_counter() - t0
+        t_used += dt
+        done += n_s
+        if t_used >= budget_s or dt <= 0:
+            break
+        n_s = int(min(max(n_s * 2, 1000), n_s * (budget_s - t_used) / dt + 1))
+        if n_s < 1000:
+            break
+    return {"value": done / t_used, "unit": "coeffs/s", "cores": threads, "kind": "port",
+            "sample": f"{done} uniformly sampled stored edges of the same graph, reference edge-centric "
+                      f"binary-search loop (jaccard.cpp:711-742) restated in C/OpenMP, {t_used:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="rmat22_weighted", choices=sorted(WORKLOADS))
+    ap.add_argument("--algo", default="ec", choices=["ec", "vc"])
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--extras", action="store_true", help="also time VC and unweighted EC")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import torch
+    from jaccard_amd import api
+
+    spec, weighted, desc = WORKLOADS[args.workload]
+    t0 = time.time()
+    g = make_graph(spec, weighted)
+    st = g.stats()
+    print(f"[bench] {desc}: n={g.n} m={g.m} sum_min={st['sum_min_deg']} max_deg={st['max_deg']} "
+          f"gen {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(0)
+    dg = api.DeviceCSR.from_host(g, dev)
+    out = torch.empty(g.m, dtype=torch.float32, device=dev)
+    W = torch.empty(g.n, dtype=torch.float32, device=dev)
+    ec = args.algo == "ec"
+    stream = torch.cuda.current_stream()
+
+    def step():
+        if weighted:
+            api.vertex_weights(dg, out=W, stream=stream)
+            api.jaccard(dg, out, weights=W, edge_centric=ec, stream=stream)
+        else:
+            api.jaccard(dg, out, edge_centric=ec, stream=stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    print("[bench] warmup done", file=sys.stderr, flush=True)
+    api.profile_reset()
+    api.profile_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    api.profile_enable(False)
+    kern = {name: api.profile_query(name) for name in api.profile_names()}
+    api.profile_reset()
+    ms_per_step = wall * 1e3 / args.steps
+    value = g.m / (wall / args.steps)
+    nb, tb = kern.get("owner_block", (0, 0.0))
+    dev_ms = sum(t for _, t in kern.values()) / args.steps
+    B = algorithmic_bytes(g, st, weighted)
+    achieved = B / (tb / nb * 1e-3) if nb else 0.0
+    print(f"[bench] {ms_per_step:.2f} ms/step, device {dev_ms:.2f} ms/step, owner_block "
+          f"{(tb / nb if nb else 0):.2f} ms", file=sys.stderr, flush=True)
+
+    extras = {"device_ms_per_step": dev_ms,
+              "kernel_ms_per_step": {k: t / args.steps for k, (c, t) in kern.items()},
+              "step_roofline_frac": B / (dev_ms * 1e-3) / HBM_PEAK if dev_ms else None,
+              "sum_min_deg": st["sum_min_deg"], "max_deg": st["max_deg"]}
+    if args.extras:
+        for name, kw in (("vc", dict(edge_centric=False)), ("ec_unweighted", dict(edge_centric=True))):
+            wts = W if (weighted and name == "vc") else None
+            api.jaccard(dg, torch.empty_like(out), weights=wts, **kw)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            for _ in range(3):
+                api.jaccard(dg, torch.empty_like(out), weights=wts, **kw)
+            torch.cuda.synchronize()
+            extras[f"{name}_ms"] = (time.perf_counter() - t1) * 1e3 / 3
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline...", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(g, args.cpu_budget)
+
+    line = {
+        "metric": METRIC, "value": value, "unit": "coeffs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64" if weighted else "int32", "data": "synthetic",
+        "config": {"workload": args.workload, "description": desc, "algorithm": "edge_centric" if ec else "vertex_centric",
+                   "n": g.n, "m": g.m, "weighted": weighted, **{k: v for k, v in spec.items() if k != "kind"}},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": None, "kernel": "k_owner_block",
+                     "algorithmic_bytes": B},
+        "cpu_baseline": cpu,
+        "extras": extras,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

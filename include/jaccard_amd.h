@@ -1,0 +1,194 @@
+/*
+ * jaccard_amd.h -- C ABI of the MI355X (gfx950) Jaccard library.
+ *
+ * Edge-connected Jaccard similarity over an int32 CSR graph, the hot path of
+ * the SYCL reference (vtsynergy/SYCL-Jaccard). Every entry point below names
+ * the reference interface it replaces (paths relative to the reference root).
+ *
+ * Conventions
+ *   - All array arguments are DEVICE pointers (hipMalloc'd or equivalent).
+ *     The caller owns inputs and outputs, as in the reference ("memory needs
+ *     to be pre-allocated by caller", standalone_algorithms.hpp:64-65).
+ *   - Calls are ordered on `stream` (NULL = the default stream) and return
+ *     without waiting for the device, except where marked "synchronous".
+ *     Temporaries are stream-ordered allocations freed on the same stream,
+ *     unless a workspace is supplied (jc_jaccard_ex).
+ *   - Return value: JC_OK (0) or a negative JC_ERR_* code; jc_strerror()
+ *     describes it. HIP runtime failures return JC_ERR_HIP and
+ *     jc_last_hip_error() gives the hipError_t of the failing call on this
+ *     thread. (The reference returns void and prints SYCL exceptions,
+ *     jaccard.cpp:178-180.)
+ *   - CSR preconditions (the reference's CSR builder guarantees them,
+ *     readMtxToCSR.cpp:152-208): offsets[0] = 0, non-decreasing,
+ *     offsets[n] = m; 0 <= indices[k] < n; columns strictly increasing inside
+ *     each row. jc_validate_csr() checks them. A symmetric CSR (every (u,v)
+ *     also stored as (v,u), main.cpp:98-112) takes the fast path; any other
+ *     CSR is still computed exactly, with a fix-up pass for unmatched edges.
+ *   - Unweighted results are bit-identical to the reference arithmetic
+ *     I / ((float)(d_u + d_v) - I). Weighted results (vertex weights W):
+ *     J = I_W / (S(u) + S(v) - I_W), S(u) = sum_{k in N(u)} W[k],
+ *     I_W = sum_{k in N(u) & N(v)} W[k], accumulated exactly (double-double)
+ *     and divided in fp64, rounded once to fp32.
+ */
+#ifndef JACCARD_AMD_H
+#define JACCARD_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Same type as hipStream_t (struct ihipStream_t *): pass a hipStream_t directly. */
+struct ihipStream_t;
+typedef struct ihipStream_t *jc_stream_t;
+
+enum {
+  JC_OK = 0,
+  JC_ERR_INVALID_ARG = -1,    /* null pointer, negative size, bad flag */
+  JC_ERR_INVALID_CSR = -2,    /* jc_validate_csr found a violated precondition */
+  JC_ERR_OUT_OF_MEMORY = -3,  /* device allocation failed */
+  JC_ERR_HIP = -4,            /* a HIP runtime call failed, see jc_last_hip_error() */
+  JC_ERR_WORKSPACE = -5       /* supplied workspace smaller than required */
+};
+
+/* Graph in CSR form: GraphCSRView<int32_t,int32_t,float>
+ * (standalone_csr.hpp:44-105; edge_data, number_of_vertices, number_of_edges,
+ * offsets, indices). All pointers are device pointers. */
+typedef struct jc_csr {
+  int32_t n;               /* number_of_vertices */
+  int32_t m;               /* number_of_edges (stored, directed) */
+  const int32_t *offsets;  /* n + 1 */
+  const int32_t *indices;  /* m */
+  const float *edge_w;     /* m edge weights, or NULL (used by jc_vertex_weights) */
+} jc_csr;
+
+#define JC_EDGE_CENTRIC 1
+#define JC_VERTEX_CENTRIC 0
+
+/* Skip the sentinel fill and fix-up pass: the caller asserts the CSR is
+ * symmetric (for instance it was checked once with jc_csr_is_symmetric). */
+#define JC_FLAG_ASSUME_SYMMETRIC 1u
+
+/* ---- Algorithm level -------------------------------------------------- */
+
+/* sygraph::jaccard<edge_centric, int, int, float>(graph, [weights,] result, q)
+ *   declarations standalone_algorithms.hpp:67-69 (weighted), :86-87
+ *   (unweighted); definitions jaccard.cpp:926-955; pipeline jaccard.cpp:784-899.
+ * vertex_weights: n vertex weights (weighted Jaccard), or NULL (unweighted).
+ * result: m outputs, one per stored edge in CSR order.
+ * edge_centric: JC_EDGE_CENTRIC (fused kernel) or JC_VERTEX_CENTRIC
+ * (row sum -> intersection -> weight kernels). Both give the same bits.
+ * Differences from the reference: the reference's edge-centric path ignores
+ * vertex weights (jaccard.cpp:790-819 always runs the unweighted kernel);
+ * here edge-centric honours them. The EC "non zero pairs" line
+ * (jaccard.cpp:811-819) is jc_nonzero_count, not a side effect. */
+int jc_jaccard(const jc_csr *g, const float *vertex_weights, float *result, int edge_centric,
+               jc_stream_t stream);
+
+/* jc_jaccard with flags (JC_FLAG_*) and an optional caller workspace of at
+ * least jc_jaccard_workspace_size() bytes (NULL: allocate per call). With a
+ * workspace the call allocates nothing. No reference counterpart: the
+ * reference allocates temporaries inside the call (jaccard.cpp:931-934). */
+int jc_jaccard_ex(const jc_csr *g, const float *vertex_weights, float *result, int edge_centric,
+                  unsigned flags, void *workspace, size_t workspace_bytes, jc_stream_t stream);
+size_t jc_jaccard_workspace_size(int32_t n, int32_t m, int weighted, int edge_centric);
+
+/* sygraph::jaccard_list<int, int, float>(graph, [weights,] num_pairs, first, second, result, q)
+ *   declarations standalone_algorithms.hpp:109-112, :131-133; definitions
+ *   jaccard.cpp:957-988 via jaccard_pairs jaccard.cpp:901-923.
+ * result[i] = J(first[i], second[i]) for any vertex pair (not only edges).
+ * Ids outside [0, n) give NaN. Same arithmetic as jc_jaccard, so the value of
+ * a stored edge matches the full-graph result bit for bit. The reference's
+ * version is compiled out (Makefile:33 -DDISABLE_LIST) and indexes weight_i
+ * by the wrong variable (jaccard.cpp:517-531); this implements the intended
+ * per-pair semantics. */
+int jc_jaccard_list(const jc_csr *g, const float *vertex_weights, int32_t num_pairs,
+                    const int32_t *first, const int32_t *second, float *result,
+                    jc_stream_t stream);
+
+/* Weighted-VC preamble of the reference driver, main.cpp:211-275:
+ * W[k] = sum of the edge weights of row k (g->edge_w, required). */
+int jc_vertex_weights(const jc_csr *g, float *W, jc_stream_t stream);
+
+/* "non zero pairs" count of the EC path, jaccard.cpp:811-819: the number of
+ * result entries greater than thresh (the reference uses 1e-5f).
+ * Synchronous: waits for the stream, writes *count on the host. */
+int jc_nonzero_count(int32_t m, const float *result, float thresh, int64_t *count,
+                     jc_stream_t stream);
+
+/* ---- Kernel level (the reference's Kernel::invoke entry points) -------- */
+
+/* FillKernel<float>::invoke, jaccard.hpp:51-52 / jaccard.cpp:161-182. */
+int jc_fill(int64_t n, float *x, float value, jc_stream_t stream);
+
+/* Jaccard_RowSumKernel<weighted>::invoke, jaccard.hpp:104-107 /
+ * jaccard.cpp:263-313: work[u] = (float)degree(u) when v is NULL, else
+ * sum_{k in row u} v[indices[k]]. Asynchronous (the reference waits, :308). */
+int jc_rowsum(int32_t n, int32_t m, const int32_t *offsets, const int32_t *indices,
+              const float *v, float *work, jc_stream_t stream);
+
+/* Jaccard_IsKernel<weighted>::invoke, jaccard.hpp:151-156 / jaccard.cpp:402-458:
+ * weight_s[j] = work[u] + work[v]; weight_i[j] += I(u, v) (v NULL: count,
+ * else sum of v over the intersection). Like the reference it adds into
+ * weight_i, so zero it first (jc_fill), as jaccard.cpp:836 does. */
+int jc_intersect(int32_t n, int32_t m, const int32_t *offsets, const int32_t *indices,
+                 const float *v, const float *work, float *weight_i, float *weight_s,
+                 jc_stream_t stream);
+
+/* Jaccard_IsPairsKernel<weighted>::invoke, jaccard.hpp:210-217 /
+ * jaccard.cpp:542-605, per pair i: weight_s[i] = work[first[i]] + work[second[i]];
+ * weight_i[i] += I(first[i], second[i]). */
+int jc_intersect_pairs(int32_t n, int32_t num_pairs, const int32_t *offsets,
+                       const int32_t *indices, const int32_t *first, const int32_t *second,
+                       const float *v, const float *work, float *weight_i, float *weight_s,
+                       jc_stream_t stream);
+
+/* Jaccard_JwKernel::invoke, jaccard.hpp:235-237 / jaccard.cpp:622-653:
+ * weight_j[j] = weight_i[j] / (weight_s[j] - weight_i[j]). */
+int jc_jw(int32_t m, const float *weight_i, const float *weight_s, float *weight_j,
+          jc_stream_t stream);
+
+/* Jaccard_ec_scan::invoke, jaccard.hpp:256-258 / jaccard.cpp:666-700:
+ * dest_ind[k] = row of edge k, weight_j[k] = 0. */
+int jc_ec_scan(int32_t m, int32_t n, const int32_t *offsets, int32_t *dest_ind, float *weight_j,
+               jc_stream_t stream);
+
+/* Jaccard_ec_unweighted::invoke, jaccard.hpp:280-283 / jaccard.cpp:745-782:
+ * weight_j[k] = I / ((float)(d_u + d_v) - I) for every stored edge. The
+ * result overwrites weight_j; dest_ind (from jc_ec_scan) is accepted for
+ * signature parity and not needed: rows are walked directly. */
+int jc_ec_unweighted(int32_t m, int32_t n, const int32_t *offsets, const int32_t *indices,
+                     const int32_t *dest_ind, float *weight_j, jc_stream_t stream);
+
+/* ---- Utilities (no reference counterpart) ------------------------------ */
+
+/* Synchronous CSR check: JC_OK or JC_ERR_INVALID_CSR. *what (optional)
+ * receives a bit mask: 1 offsets, 2 column range, 4 row order. */
+int jc_validate_csr(const jc_csr *g, int *what, jc_stream_t stream);
+
+/* Synchronous symmetry check: *symmetric = 1 if every stored (u,v) has (v,u). */
+int jc_csr_is_symmetric(const jc_csr *g, int *symmetric, jc_stream_t stream);
+
+/* Per-kernel device timing, the counterpart of the reference's EVENT_PROFILE
+ * build (wait_and_print, standalone_algorithms.hpp:37-47, used at
+ * jaccard.cpp:798-809, 875-896): while enabled, each kernel launch of the
+ * library is bracketed by hipEvents on its stream. jc_profile_query
+ * (synchronous) sums the launches of one kernel name ("owner_block",
+ * "owner_small", "sched", "fixup", "fill", "rowsum", "jw", "vertex_weights",
+ * "ec_scan", "pairs", "count_above"; NULL = all). */
+int jc_profile_enable(int on);
+int jc_profile_reset(void);
+int jc_profile_query(const char *kernel, int *count, double *total_ms);
+int jc_profile_names(char *buf, size_t len);
+
+const char *jc_strerror(int code);
+int jc_last_hip_error(void);
+const char *jc_version(void);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* JACCARD_AMD_H */

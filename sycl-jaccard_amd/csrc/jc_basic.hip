@@ -1,0 +1,285 @@
+// Streaming kernels of the vertex-centric pipeline and the EC row expansion.
+//
+//   k_fill             FillKernel            jaccard.cpp:150-182
+//   k_rowsum_*         Jaccard_RowSumKernel  jaccard.cpp:235-313 (+ parallel_prefix_sum :53-137)
+//   k_vertex_weights   weighted-VC preamble  main.cpp:211-275 (W(k) = sum of row k's edge weights)
+//   k_jw_*             Jaccard_JwKernel      jaccard.cpp:607-653
+//   k_ec_scan          Jaccard_ec_scan       jaccard.cpp:654-700
+//   k_count_above      the "non zero pairs" count  jaccard.cpp:811-819
+//
+// Row sums are one wave per row: the row is read with coalesced 256-byte loads
+// and reduced with an xor butterfly (double-double, exact for fp32 addends of
+// moderate dynamic range) instead of the reference's LDS-emulated 32-lane
+// shuffle scan with two barriers per step.
+#include "jc_device.hpp"
+#include "jc_internal.hpp"
+#include "jc_profile.hpp"
+
+namespace jc {
+
+namespace {
+
+struct DD2 {
+  double hi, lo;
+};
+
+__device__ __forceinline__ DD2 add_dd(DD2 a, DD2 b) {
+  const double s = a.hi + b.hi;
+  const double bb = s - a.hi;
+  const double e = (a.hi - (s - bb)) + (b.hi - bb);
+  double t = a.lo + b.lo;
+  t += e;
+  const double hi = s + t;
+  return DD2{hi, t - (hi - s)};
+}
+
+__device__ __forceinline__ DD2 add_d(DD2 a, double v) {
+  const double s = a.hi + v;
+  const double bb = s - a.hi;
+  const double e = (a.hi - (s - bb)) + (v - bb);
+  const double t = a.lo + e;
+  const double hi = s + t;
+  return DD2{hi, t - (hi - s)};
+}
+
+__device__ __forceinline__ double wave_reduce_dd(DD2 v) {
+#pragma unroll
+  for (int s = 1; s < kWave; s <<= 1) v = add_dd(v, DD2{__shfl_xor(v.hi, s), __shfl_xor(v.lo, s)});
+  return v.hi;
+}
+
+int grid_for(int64_t items, int per_block, int cap_per_cu) {
+  const int64_t g = (items + per_block - 1) / per_block;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(g, (int64_t)device_cu_count() * cap_per_cu));
+}
+
+}  // namespace
+
+template <typename T>
+__global__ void k_fill(T* __restrict__ x, int64_t n, T v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    x[i] = v;
+}
+
+hipError_t launch_fill_f32(float* x, int64_t n, float v, hipStream_t s) {
+  ProfScope ps_("fill", s);
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill<float>, dim3(grid_for(n, 256 * 4, 32)), dim3(256), 0, s, x, n, v);
+  return hipGetLastError();
+}
+hipError_t launch_fill_u32(uint32_t* x, int64_t n, uint32_t v, hipStream_t s) {
+  ProfScope ps_("fill", s);
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill<uint32_t>, dim3(grid_for(n, 256 * 4, 32)), dim3(256), 0, s, x, n, v);
+  return hipGetLastError();
+}
+hipError_t launch_fill_u64(uint64_t* x, int64_t n, uint64_t v, hipStream_t s) {
+  ProfScope ps_("fill", s);
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill<uint64_t>, dim3(grid_for(n, 256 * 4, 32)), dim3(256), 0, s, x, n, v);
+  return hipGetLastError();
+}
+
+// Unweighted row sum: work[row] = (float)degree (jaccard.cpp:257-258).
+__global__ void k_rowsum_deg(int32_t n, const int32_t* __restrict__ off, float* __restrict__ work) {
+  for (int32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x)
+    work[u] = (float)(off[u + 1] - off[u]);
+}
+
+// Weighted row sum over column weights: sum_{k in row u} v[ind[k]] (ind may be
+// nullptr: identity index, i.e. the row's own edge weights, as main.cpp:216-257
+// arranges with its presumInd buffer).
+template <typename OutT>
+__global__ void k_rowsum_w(int32_t n, const int32_t* __restrict__ off, const int32_t* __restrict__ ind,
+                           const float* __restrict__ v, OutT* __restrict__ out) {
+  const int nw = gridDim.x * (blockDim.x / kWave);
+  for (int32_t u = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; u < n; u += nw) {
+    const int o = off[u], d = off[u + 1] - o;
+    DD2 acc{0.0, 0.0};
+    for (int p = o + lane_id(); p < o + d; p += kWave) acc = add_d(acc, (double)v[ind ? ind[p] : p]);
+    const double s = wave_reduce_dd(acc);
+    if (lane_id() == 0) out[u] = (OutT)s;
+  }
+}
+
+hipError_t launch_rowsum_f32(int32_t n, const int32_t* off, const int32_t* ind, const float* v,
+                             float* work, hipStream_t s) {
+  ProfScope ps_("rowsum", s);
+  if (n <= 0) return hipSuccess;
+  if (v == nullptr)
+    hipLaunchKernelGGL(k_rowsum_deg, dim3(grid_for(n, 256, 32)), dim3(256), 0, s, n, off, work);
+  else
+    hipLaunchKernelGGL(k_rowsum_w<float>, dim3(grid_for(n, 4, 64)), dim3(256), 0, s, n, off, ind, v, work);
+  return hipGetLastError();
+}
+
+hipError_t launch_rowsum_f64(int32_t n, const int32_t* off, const int32_t* ind, const float* v,
+                             double* S, hipStream_t s) {
+  ProfScope ps_("rowsum", s);
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_rowsum_w<double>, dim3(grid_for(n, 4, 64)), dim3(256), 0, s, n, off, ind, v, S);
+  return hipGetLastError();
+}
+
+hipError_t launch_vertex_weights(int32_t n, const int32_t* off, const float* edge_w, float* W,
+                                 hipStream_t s) {
+  ProfScope ps_("vertex_weights", s);
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_rowsum_w<float>, dim3(grid_for(n, 4, 64)), dim3(256), 0, s, n, off,
+                     (const int32_t*)nullptr, edge_w, W);
+  return hipGetLastError();
+}
+
+// Jaccard weight: weight_j = Wi / (Ws - Wi) (jaccard.cpp:614-619).
+__global__ void k_jw_f32(int64_t m, const float* __restrict__ wi, const float* __restrict__ ws,
+                         float* __restrict__ out) {
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m; j += (int64_t)gridDim.x * blockDim.x) {
+    const float Wi = wi[j];
+    const float Ws = ws[j];
+    out[j] = Wi / (Ws - Wi);
+  }
+}
+
+__global__ void k_jw_f64(int64_t m, const double* __restrict__ wi, const double* __restrict__ ws,
+                         float* __restrict__ out) {
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m; j += (int64_t)gridDim.x * blockDim.x) {
+    const double Wi = wi[j];
+    const double Ws = ws[j];
+    out[j] = (float)(Wi / (Ws - Wi));
+  }
+}
+
+hipError_t launch_jw_f32(int64_t m, const float* wi, const float* ws, float* out, hipStream_t s) {
+  ProfScope ps_("jw", s);
+  if (m <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_jw_f32, dim3(grid_for(m, 256 * 4, 32)), dim3(256), 0, s, m, wi, ws, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_jw_f64(int64_t m, const double* wi, const double* ws, float* out, hipStream_t s) {
+  ProfScope ps_("jw", s);
+  if (m <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_jw_f64, dim3(grid_for(m, 256 * 4, 32)), dim3(256), 0, s, m, wi, ws, out);
+  return hipGetLastError();
+}
+
+// Row expansion: dest[k] = row(k), weight_j[k] = 0. One wave per row, so the
+// writes of a row are coalesced (the reference writes a row with one thread).
+__global__ void k_ec_scan(int32_t n, const int32_t* __restrict__ off, int32_t* __restrict__ dest,
+                          float* __restrict__ wj) {
+  const int nw = gridDim.x * (blockDim.x / kWave);
+  for (int32_t u = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; u < n; u += nw) {
+    const int o = off[u], e = off[u + 1];
+    for (int p = o + lane_id(); p < e; p += kWave) {
+      dest[p] = u;
+      wj[p] = 0.0f;
+    }
+  }
+}
+
+hipError_t launch_ec_scan(int32_t n, const int32_t* off, int32_t* dest, float* wj, hipStream_t s) {
+  ProfScope ps_("ec_scan", s);
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_ec_scan, dim3(grid_for(n, 4, 64)), dim3(256), 0, s, n, off, dest, wj);
+  return hipGetLastError();
+}
+
+__global__ void k_count_above(int64_t m, const float* __restrict__ x, float thresh,
+                              unsigned long long* cnt) {
+  unsigned long long c = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
+    c += x[i] > thresh ? 1u : 0u;
+  for (int s = 1; s < kWave; s <<= 1) c += __shfl_xor(c, s);
+  if (lane_id() == 0 && c) atomicAdd(cnt, c);
+}
+
+hipError_t launch_count_above(int64_t m, const float* x, float thresh, unsigned long long* cnt,
+                              hipStream_t s) {
+  ProfScope ps_("count_above", s);
+  if (m <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_count_above, dim3(grid_for(m, 256 * 8, 16)), dim3(256), 0, s, m, x, thresh, cnt);
+  return hipGetLastError();
+}
+
+// CSR validation. flags[0]: offsets bad (offsets[0] != 0, decreasing, or
+// offsets[n] != m, or out of [0, m]); flags[1]: a column out of [0, n);
+// flags[2]: a row not strictly increasing (unsorted or duplicate columns).
+// Every read is bounds-checked before it is made.
+__global__ void k_validate(int32_t n, int64_t m, const int32_t* __restrict__ off,
+                           const int32_t* __restrict__ ind, int* flags) {
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nt = (int64_t)gridDim.x * blockDim.x;
+  if (tid == 0 && (off[0] != 0 || (int64_t)off[n] != m)) atomicOr(&flags[0], 1);
+  for (int64_t u = tid; u < n; u += nt) {
+    const int32_t a = off[u], b = off[u + 1];
+    if (a < 0 || b < a || (int64_t)b > m) atomicOr(&flags[0], 1);
+  }
+  for (int64_t k = tid; k < m; k += nt) {
+    const int32_t c = ind[k];
+    if (c < 0 || c >= n) atomicOr(&flags[1], 1);
+  }
+}
+
+// Second pass (only run when the offsets are valid): sortedness per row.
+__global__ void k_validate_rows(int32_t n, const int32_t* __restrict__ off,
+                                const int32_t* __restrict__ ind, int* flags) {
+  const int nw = gridDim.x * (blockDim.x / kWave);
+  for (int32_t u = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; u < n; u += nw) {
+    const int o = off[u], e = off[u + 1];
+    bool bad = false;
+    for (int p = o + 1 + lane_id(); p < e; p += kWave) bad |= ind[p] <= ind[p - 1];
+    if (ballot(bad) && lane_id() == 0) atomicOr(&flags[2], 1);
+  }
+}
+
+hipError_t launch_validate(int32_t n, int64_t m, const int32_t* off, const int32_t* ind, int* flags,
+                           hipStream_t s) {
+  hipLaunchKernelGGL(k_validate, dim3(grid_for(std::max<int64_t>(n, m), 256 * 4, 16)), dim3(256), 0, s, n, m,
+                     off, ind, flags);
+  return hipGetLastError();
+}
+
+hipError_t launch_validate_rows(int32_t n, const int32_t* off, const int32_t* ind, int* flags,
+                                hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_validate_rows, dim3(grid_for(n, 4, 64)), dim3(256), 0, s, n, off, ind, flags);
+  return hipGetLastError();
+}
+
+// Symmetry check: flag if some stored (u, v) has no stored (v, u).
+__global__ void k_symcheck(int32_t n, const int32_t* __restrict__ off, const int32_t* __restrict__ ind,
+                           int* flag) {
+  const int nw = gridDim.x * (blockDim.x / kWave);
+  for (int32_t u = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; u < n; u += nw) {
+    bool bad = false;
+    for (int p = off[u] + lane_id(); p < off[u + 1]; p += kWave) {
+      const int32_t v = ind[p];
+      const int lo = off[v], hi = off[v + 1];
+      const int q = lane_lower_bound(ind, lo, hi, u);
+      bad |= !(q < hi && ind[q] == u);
+    }
+    if (ballot(bad) && lane_id() == 0) atomicOr(flag, 1);
+  }
+}
+
+hipError_t launch_symcheck(int32_t n, int64_t m, const int32_t* off, const int32_t* ind, int* flag,
+                           hipStream_t s) {
+  (void)m;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_symcheck, dim3(grid_for(n, 4, 64)), dim3(256), 0, s, n, off, ind, flag);
+  return hipGetLastError();
+}
+
+int device_cu_count() {
+  static int cached[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (cached[dev] == 0) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
+    cached[dev] = v;
+  }
+  return cached[dev];
+}
+
+}  // namespace jc

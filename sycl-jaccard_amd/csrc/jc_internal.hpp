@@ -1,0 +1,81 @@
+// Internal interfaces between the C ABI layer and the kernel translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace jc {
+
+// Epilogue of an intersection: what is written for a pair (u, x).
+enum Mode : int {
+  kModeEcU = 0,   // EC unweighted: out = I / ((float)(du+dx) - I)
+  kModeEcW = 1,   // EC weighted:   out = (float)(I / ((S[u]+S[x]) - I)), fp64
+  kModeIsF32 = 2, // VC intersection, reference buffers: weight_i (+)= I, weight_s = work[u]+work[x]
+  kModeIsF64 = 3, // VC intersection, fp64 internals: wi64 = I, ws64 = S[u]+S[x]
+};
+
+// Rows up to this degree are handled one wave per row; longer rows by a
+// workgroup per task. Tasks cover at most kTaskSpan consecutive entries of a
+// row.
+constexpr int kSmallMax = 128;
+constexpr int kTaskSpan = 512;
+
+struct EngineArgs {
+  int32_t n;
+  int64_t m;
+  const int32_t* off;
+  const int32_t* ind;
+  const float* vw;       // vertex weights (weighted modes), else nullptr
+  const double* S64;     // fp64 row sums of vw (kModeEcW, kModeIsF64)
+  const float* work32;   // fp32 row sums (kModeIsF32)
+  float* out32;          // EC result, or weight_i (kModeIsF32)
+  float* ws32;           // weight_s (kModeIsF32)
+  double* wi64;          // kModeIsF64
+  double* ws64;          // kModeIsF64
+  int accumulate;        // kModeIsF32: add I to the prior weight_i contents
+  // schedule (device memory)
+  const int2* tasks;     // (row, piece) for rows with degree > kSmallMax, degree-descending buckets
+  const int* num_tasks;
+  int* task_counter;
+  const int32_t* small_rows;  // rows with 1 <= degree <= kSmallMax
+  const int* num_small;
+};
+
+// Workspace needed by an owner-engine pass over a graph with n vertices and m
+// stored edges (task list, small-row list, counters).
+size_t engine_workspace_bytes(int32_t n, int64_t m);
+
+// Owner engine: every pair written once per stored direction. When
+// check_sentinel is set the target array was pre-filled with the sentinel and
+// a fix-up pass recomputes positions that no symmetric pair reached.
+// ws must hold engine_workspace_bytes(n, m) bytes.
+hipError_t run_engine(int mode, bool weighted, const EngineArgs& base, void* ws, bool fixup,
+                      hipStream_t stream);
+
+// Pair path: out per (first[i], second[i]); invalid ids give NaN.
+hipError_t run_pairs(int mode, bool weighted, int32_t n, const int32_t* off, const int32_t* ind,
+                     const float* vw, const float* work32, int64_t np, const int32_t* first,
+                     const int32_t* second, float* out32, float* ws32, int accumulate,
+                     hipStream_t stream);
+
+// Streaming kernels.
+hipError_t launch_fill_f32(float* x, int64_t n, float v, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t* x, int64_t n, uint32_t v, hipStream_t s);
+hipError_t launch_fill_u64(uint64_t* x, int64_t n, uint64_t v, hipStream_t s);
+hipError_t launch_rowsum_f32(int32_t n, const int32_t* off, const int32_t* ind, const float* v,
+                             float* work, hipStream_t s);
+hipError_t launch_rowsum_f64(int32_t n, const int32_t* off, const int32_t* ind, const float* v,
+                             double* S, hipStream_t s);
+hipError_t launch_vertex_weights(int32_t n, const int32_t* off, const float* edge_w, float* W,
+                                 hipStream_t s);
+hipError_t launch_jw_f32(int64_t m, const float* wi, const float* ws, float* out, hipStream_t s);
+hipError_t launch_jw_f64(int64_t m, const double* wi, const double* ws, float* out, hipStream_t s);
+hipError_t launch_ec_scan(int32_t n, const int32_t* off, int32_t* dest, float* wj, hipStream_t s);
+hipError_t launch_count_above(int64_t m, const float* x, float thresh, unsigned long long* cnt,
+                              hipStream_t s);
+hipError_t launch_validate(int32_t n, int64_t m, const int32_t* off, const int32_t* ind,
+                           int* flags, hipStream_t s);
+
+int device_cu_count();
+
+}  // namespace jc

@@ -1,0 +1,256 @@
+"""Python mirror of the reference's Jaccard interface over the C ABI.
+
+Reference interface (paths relative to the reference root):
+  * ``sygraph::jaccard<edge_centric,VT,ET,WT>(graph, [weights,] result, q)``
+    standalone_algorithms.hpp:67-69, :86-87
+  * ``sygraph::jaccard_list<VT,ET,WT>(graph, [weights,] num_pairs, first, second, result, q)``
+    standalone_algorithms.hpp:109-112, :131-133
+  * the kernel functors' ``invoke`` (jaccard.hpp:51-52, 104-107, 151-156,
+    210-217, 235-237, 256-258, 280-283) and the driver's vertex-weight
+    preamble (main.cpp:211-275)
+
+Device memory and streams are torch tensors / torch streams (plumbing only);
+all arithmetic runs in ``libjaccard_amd.so``. Errors raise ``JaccardError``
+carrying the library's status code, where the reference printed a SYCL
+exception and carried on.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional
+
+from ._native import load_jaccard_lib
+
+EDGE_CENTRIC = 1
+VERTEX_CENTRIC = 0
+FLAG_ASSUME_SYMMETRIC = 1
+
+ERR_INVALID_ARG = -1
+ERR_INVALID_CSR = -2
+ERR_OUT_OF_MEMORY = -3
+ERR_HIP = -4
+ERR_WORKSPACE = -5
+
+
+class JaccardError(RuntimeError):
+    def __init__(self, code: int, what: str):
+        lib = load_jaccard_lib()
+        msg = lib.jc_strerror(code).decode()
+        if code == ERR_HIP:
+            msg += f" (hipError {lib.jc_last_hip_error()})"
+        super().__init__(f"{what}: {msg} [{code}]")
+        self.code = code
+
+
+class JcCsr(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int32), ("m", ctypes.c_int32), ("offsets", ctypes.c_void_p),
+                ("indices", ctypes.c_void_p), ("edge_w", ctypes.c_void_p)]
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _ptr(t) -> Optional[int]:
+    return None if t is None else t.data_ptr()
+
+
+def _stream(stream) -> Optional[int]:
+    torch = _torch()
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream)
+
+
+def _check(code: int, what: str) -> None:
+    if code != 0:
+        raise JaccardError(code, what)
+
+
+class DeviceCSR:
+    """``GraphCSRView<int32_t,int32_t,float>`` resident on the GPU.
+
+    ``offsets`` (int32[n+1]), ``indices`` (int32[m]) and optional ``edge_w``
+    (float32[m]) are device tensors owned by this object.
+    """
+
+    def __init__(self, n: int, offsets, indices, edge_w=None):
+        torch = _torch()
+        for t, dt, name in ((offsets, torch.int32, "offsets"), (indices, torch.int32, "indices")):
+            if t.dtype != dt or not t.is_cuda or not t.is_contiguous():
+                raise ValueError(f"{name} must be a contiguous int32 CUDA tensor")
+        if offsets.numel() != n + 1:
+            raise ValueError("offsets must have n+1 entries")
+        if edge_w is not None and (edge_w.dtype != torch.float32 or edge_w.numel() != indices.numel()):
+            raise ValueError("edge_w must be float32[m]")
+        self.n = int(n)
+        self.offsets = offsets
+        self.indices = indices
+        self.edge_w = edge_w
+
+    @property
+    def m(self) -> int:
+        return int(self.indices.numel())
+
+    @classmethod
+    def from_host(cls, g, device="cuda") -> "DeviceCSR":
+        torch = _torch()
+        off = torch.from_numpy(g.offsets).to(device)
+        ind = torch.from_numpy(g.indices).to(device) if g.m else torch.zeros(0, dtype=torch.int32, device=device)
+        w = torch.from_numpy(g.weights).to(device) if g.weights is not None else None
+        return cls(g.n, off, ind, w)
+
+    def c_struct(self) -> JcCsr:
+        return JcCsr(self.n, self.m, _ptr(self.offsets), _ptr(self.indices) if self.m else None,
+                     _ptr(self.edge_w) if self.edge_w is not None and self.m else None)
+
+
+def _empty_f32(n, device):
+    torch = _torch()
+    return torch.empty(max(n, 0), dtype=torch.float32, device=device)
+
+
+def jaccard(graph: DeviceCSR, result=None, weights=None, edge_centric: bool = True, stream=None,
+            flags: int = 0, workspace=None):
+    """Jaccard coefficient of every stored edge (``sygraph::jaccard``).
+
+    ``weights``: float32[n] vertex weights, or None for the unweighted form.
+    ``result``: float32[m] output (allocated when None). Returns ``result``.
+    """
+    lib = load_jaccard_lib()
+    if result is None:
+        result = _empty_f32(graph.m, graph.offsets.device)
+    g = graph.c_struct()
+    ws_ptr, ws_bytes = (None, 0) if workspace is None else (workspace.data_ptr(), workspace.numel() * workspace.element_size())
+    _check(lib.jc_jaccard_ex(ctypes.byref(g), _ptr(weights), _ptr(result) if graph.m else None,
+                             EDGE_CENTRIC if edge_centric else VERTEX_CENTRIC, flags, ws_ptr, ws_bytes,
+                             _stream(stream)), "jc_jaccard")
+    return result
+
+
+def workspace_size(n: int, m: int, weighted: bool, edge_centric: bool) -> int:
+    return int(load_jaccard_lib().jc_jaccard_workspace_size(n, m, int(weighted), int(edge_centric)))
+
+
+def jaccard_list(graph: DeviceCSR, first, second, result=None, weights=None, stream=None):
+    """Jaccard coefficient of arbitrary vertex pairs (``sygraph::jaccard_list``)."""
+    lib = load_jaccard_lib()
+    k = int(first.numel())
+    if second.numel() != k:
+        raise ValueError("first and second must have the same length")
+    if result is None:
+        result = _empty_f32(k, graph.offsets.device)
+    g = graph.c_struct()
+    _check(lib.jc_jaccard_list(ctypes.byref(g), _ptr(weights), k, _ptr(first), _ptr(second), _ptr(result),
+                               _stream(stream)), "jc_jaccard_list")
+    return result
+
+
+def vertex_weights(graph: DeviceCSR, out=None, stream=None):
+    """W[k] = sum of row k's edge weights (the reference driver's preamble)."""
+    lib = load_jaccard_lib()
+    if out is None:
+        out = _empty_f32(graph.n, graph.offsets.device)
+    g = graph.c_struct()
+    _check(lib.jc_vertex_weights(ctypes.byref(g), _ptr(out), _stream(stream)), "jc_vertex_weights")
+    return out
+
+
+def nonzero_count(result, thresh: float = 1e-5, stream=None) -> int:
+    """Entries above ``thresh`` (the EC path's "non zero pairs" line)."""
+    lib = load_jaccard_lib()
+    c = ctypes.c_int64(0)
+    _check(lib.jc_nonzero_count(int(result.numel()), _ptr(result), float(thresh), ctypes.byref(c),
+                                _stream(stream)), "jc_nonzero_count")
+    return int(c.value)
+
+
+# ---- kernel level ----------------------------------------------------------
+
+def fill(x, value: float, stream=None):
+    _check(load_jaccard_lib().jc_fill(int(x.numel()), _ptr(x), float(value), _stream(stream)), "jc_fill")
+    return x
+
+
+def rowsum(graph: DeviceCSR, work, v=None, stream=None):
+    _check(load_jaccard_lib().jc_rowsum(graph.n, graph.m, _ptr(graph.offsets), _ptr(graph.indices), _ptr(v),
+                                        _ptr(work), _stream(stream)), "jc_rowsum")
+    return work
+
+
+def intersect(graph: DeviceCSR, work, weight_i, weight_s, v=None, stream=None):
+    _check(load_jaccard_lib().jc_intersect(graph.n, graph.m, _ptr(graph.offsets), _ptr(graph.indices), _ptr(v),
+                                           _ptr(work), _ptr(weight_i), _ptr(weight_s), _stream(stream)),
+           "jc_intersect")
+
+
+def intersect_pairs(graph: DeviceCSR, first, second, work, weight_i, weight_s, v=None, stream=None):
+    _check(load_jaccard_lib().jc_intersect_pairs(graph.n, int(first.numel()), _ptr(graph.offsets),
+                                                 _ptr(graph.indices), _ptr(first), _ptr(second), _ptr(v),
+                                                 _ptr(work), _ptr(weight_i), _ptr(weight_s), _stream(stream)),
+           "jc_intersect_pairs")
+
+
+def jw(weight_i, weight_s, weight_j, stream=None):
+    _check(load_jaccard_lib().jc_jw(int(weight_i.numel()), _ptr(weight_i), _ptr(weight_s), _ptr(weight_j),
+                                    _stream(stream)), "jc_jw")
+    return weight_j
+
+
+def ec_scan(graph: DeviceCSR, dest_ind, weight_j, stream=None):
+    _check(load_jaccard_lib().jc_ec_scan(graph.m, graph.n, _ptr(graph.offsets), _ptr(dest_ind), _ptr(weight_j),
+                                         _stream(stream)), "jc_ec_scan")
+
+
+def ec_unweighted(graph: DeviceCSR, dest_ind, weight_j, stream=None):
+    _check(load_jaccard_lib().jc_ec_unweighted(graph.m, graph.n, _ptr(graph.offsets), _ptr(graph.indices),
+                                               _ptr(dest_ind), _ptr(weight_j), _stream(stream)),
+           "jc_ec_unweighted")
+    return weight_j
+
+
+def validate_csr(graph: DeviceCSR, stream=None) -> int:
+    """0 when valid, else a bit mask: 1 offsets, 2 column range, 4 row order."""
+    lib = load_jaccard_lib()
+    what = ctypes.c_int(0)
+    g = graph.c_struct()
+    code = lib.jc_validate_csr(ctypes.byref(g), ctypes.byref(what), _stream(stream))
+    if code not in (0, ERR_INVALID_CSR):
+        raise JaccardError(code, "jc_validate_csr")
+    return int(what.value)
+
+
+def is_symmetric(graph: DeviceCSR, stream=None) -> bool:
+    lib = load_jaccard_lib()
+    sym = ctypes.c_int(0)
+    g = graph.c_struct()
+    _check(lib.jc_csr_is_symmetric(ctypes.byref(g), ctypes.byref(sym), _stream(stream)), "jc_csr_is_symmetric")
+    return bool(sym.value)
+
+
+def profile_enable(on: bool = True) -> None:
+    """Per-kernel event timing (the reference's EVENT_PROFILE counterpart)."""
+    load_jaccard_lib().jc_profile_enable(1 if on else 0)
+
+
+def profile_reset() -> None:
+    load_jaccard_lib().jc_profile_reset()
+
+
+def profile_query(kernel: Optional[str] = None):
+    """(launch count, total ms) of one kernel name, or of all when None. Synchronous."""
+    c, t = ctypes.c_int(0), ctypes.c_double(0.0)
+    _check(load_jaccard_lib().jc_profile_query(kernel.encode() if kernel else None, ctypes.byref(c),
+                                               ctypes.byref(t)), "jc_profile_query")
+    return int(c.value), float(t.value)
+
+
+def profile_names():
+    buf = ctypes.create_string_buffer(4096)
+    load_jaccard_lib().jc_profile_names(buf, 4096)
+    return [x for x in buf.value.decode().split(",") if x]
+
+
+def version() -> str:
+    return load_jaccard_lib().jc_version().decode()

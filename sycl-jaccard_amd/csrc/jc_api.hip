@@ -48,7 +48,7 @@ bool csr_args_ok(const jc_csr* g) {
 
 // Layout of the jc_jaccard_ex workspace.
 struct Layout {
-  size_t engine, s64, work32, wi, ws, total;
+  size_t engine, rows, s64, work32, wi, ws, total;
 };
 
 Layout layout(int32_t n, int32_t m, bool weighted, bool ec) {
@@ -56,6 +56,8 @@ Layout layout(int32_t n, int32_t m, bool weighted, bool ec) {
   size_t o = 0;
   L.engine = o;
   o += al(jc::engine_workspace_bytes(n, m));
+  L.rows = o;
+  if (weighted) o += al(jc::rowsum_scratch_bytes(n));
   L.s64 = o;
   if (weighted) o += al((size_t)std::max(n, 1) * sizeof(double));
   L.work32 = o;
@@ -89,7 +91,7 @@ int run_jaccard(const jc_csr* g, const float* vw, float* result, bool ec, unsign
   a.ind = g->indices;
   a.vw = vw;
   double* S64 = weighted ? reinterpret_cast<double*>(ws + L.s64) : nullptr;
-  if (weighted) JC_TRY(jc::launch_rowsum_f64(g->n, g->offsets, g->indices, vw, S64, s));
+  if (weighted) JC_TRY(jc::launch_rowsum_f64(g->n, g->offsets, g->indices, vw, S64, ws + L.rows, s));
   a.S64 = S64;
   if (ec) {
     a.out32 = result;
@@ -110,7 +112,7 @@ int run_jaccard(const jc_csr* g, const float* vw, float* result, bool ec, unsign
   float* work = reinterpret_cast<float*>(ws + L.work32);
   float* wi = reinterpret_cast<float*>(ws + L.wi);
   float* wsum = reinterpret_cast<float*>(ws + L.ws);
-  JC_TRY(jc::launch_rowsum_f32(g->n, g->offsets, g->indices, nullptr, work, s));
+  JC_TRY(jc::launch_rowsum_f32(g->n, g->offsets, g->indices, nullptr, work, nullptr, s));
   a.work32 = work;
   a.out32 = wi;
   a.ws32 = wsum;
@@ -170,7 +172,7 @@ int jc_vertex_weights(const jc_csr* g, float* W, jc_stream_t stream) {
     JC_TRY(jc::launch_fill_f32(W, g->n, 0.0f, hs(stream)));
     return JC_OK;
   }
-  JC_TRY(jc::launch_vertex_weights(g->n, g->offsets, g->edge_w, W, hs(stream)));
+  JC_TRY(jc::launch_vertex_weights(g->n, g->offsets, g->edge_w, W, nullptr, hs(stream)));
   return JC_OK;
 }
 
@@ -201,7 +203,7 @@ int jc_fill(int64_t n, float* x, float value, jc_stream_t stream) {
 int jc_rowsum(int32_t n, int32_t m, const int32_t* offsets, const int32_t* indices, const float* v,
               float* work, jc_stream_t stream) {
   if (n < 0 || m < 0 || (n > 0 && (!offsets || !work)) || (m > 0 && v && !indices)) return JC_ERR_INVALID_ARG;
-  JC_TRY(jc::launch_rowsum_f32(n, offsets, indices, v, work, hs(stream)));
+  JC_TRY(jc::launch_rowsum_f32(n, offsets, indices, v, work, nullptr, hs(stream)));
   return JC_OK;
 }
 

@@ -53,6 +53,13 @@ int grid_for(int64_t items, int per_block, int cap_per_cu) {
   return (int)std::max<int64_t>(1, std::min<int64_t>(g, (int64_t)device_cu_count() * cap_per_cu));
 }
 
+// Grid for kernels that stride over rows with one wave per row: an odd
+// workgroup count, so rows at ids with many trailing zero bits (the hubs of
+// RMAT-like graphs) do not all land on the same few waves.
+int grid_rows(int64_t rows, int waves_per_block, int cap_per_cu) {
+  return grid_for(rows, waves_per_block, cap_per_cu) | 1;
+}
+
 }  // namespace
 
 template <typename T>
@@ -88,46 +95,162 @@ __global__ void k_rowsum_deg(int32_t n, const int32_t* __restrict__ off, float* 
 
 // Weighted row sum over column weights: sum_{k in row u} v[ind[k]] (ind may be
 // nullptr: identity index, i.e. the row's own edge weights, as main.cpp:216-257
-// arranges with its presumInd buffer).
-template <typename OutT>
+// arranges with its presumInd buffer). Rows up to kRowWave entries: one wave
+// per row; longer rows: one workgroup per row (second kernel), so a hub row is
+// not a serial tail. Loads are unrolled 4x per lane.
+#ifndef JC_ROW_WAVE
+#define JC_ROW_WAVE 2048
+#endif
+constexpr int kRowWave = JC_ROW_WAVE;
+constexpr int kRowU = 4;
+
+// Loads of one unrolled step are unconditional (clamped index) so they are
+// all in flight together; IDX selects v[ind[p]] (column weights) or v[p]
+// (the row's own edge weights).
+template <bool IDX, int U>
+__device__ __forceinline__ void gather_step(const int32_t* __restrict__ ind, const float* __restrict__ v, int p0,
+                                            int stride, int lane, int end, float (&x)[U]) {
+  int q[U];
+#pragma unroll
+  for (int t = 0; t < U; ++t) q[t] = min(p0 + t * stride + lane, end - 1);
+  if constexpr (IDX) {
+#pragma unroll
+    for (int t = 0; t < U; ++t) q[t] = ind[q[t]];
+  }
+#pragma unroll
+  for (int t = 0; t < U; ++t) x[t] = v[q[t]];
+#pragma unroll
+  for (int t = 0; t < U; ++t)
+    if (p0 + t * stride + lane >= end) x[t] = 0.0f;
+}
+
+template <typename OutT, bool IDX>
 __global__ void k_rowsum_w(int32_t n, const int32_t* __restrict__ off, const int32_t* __restrict__ ind,
                            const float* __restrict__ v, OutT* __restrict__ out) {
   const int nw = gridDim.x * (blockDim.x / kWave);
   for (int32_t u = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; u < n; u += nw) {
     const int o = off[u], d = off[u + 1] - o;
+    if (d > kRowWave) continue;
     DD2 acc{0.0, 0.0};
-    for (int p = o + lane_id(); p < o + d; p += kWave) acc = add_d(acc, (double)v[ind ? ind[p] : p]);
+    for (int p0 = o; p0 < o + d; p0 += kWave * kRowU) {
+      float x[kRowU];
+      gather_step<IDX, kRowU>(ind, v, p0, kWave, lane_id(), o + d, x);
+#pragma unroll
+      for (int t = 0; t < kRowU; ++t) acc = add_d(acc, (double)x[t]);
+    }
     const double s = wave_reduce_dd(acc);
     if (lane_id() == 0) out[u] = (OutT)s;
   }
 }
 
+// Rows longer than kRowWave: collected into a list, then taken one at a time
+// by persistent 1024-thread workgroups from an atomic counter. (A static
+// row-to-workgroup map is not used: in power-law graphs the long rows sit at
+// structured ids -- RMAT hubs have many trailing zero bits -- and any fixed
+// stride piles them onto a few workgroups.)
+__global__ void k_collect_long(int32_t n, const int32_t* __restrict__ off, int* __restrict__ list,
+                               int* __restrict__ counters) {
+  for (int32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x)
+    if (off[u + 1] - off[u] > kRowWave) list[atomicAdd(&counters[0], 1)] = u;
+}
+
+template <typename OutT, bool IDX, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_rowsum_w_long(const int32_t* __restrict__ off,
+                                                         const int32_t* __restrict__ ind,
+                                                         const float* __restrict__ v, OutT* __restrict__ out,
+                                                         const int* __restrict__ list, int* counters) {
+  __shared__ double part[2][BLOCK / kWave];
+  __shared__ int s_row;
+  const int count = counters[0];
+  for (;;) {
+    if (threadIdx.x == 0) {
+      const int r = atomicAdd(&counters[1], 1);
+      s_row = r < count ? list[r] : -1;
+    }
+    __syncthreads();
+    const int32_t u = s_row;
+    if (u < 0) return;
+    const int o = off[u], d = off[u + 1] - o;
+    DD2 acc{0.0, 0.0};
+    for (int p0 = o; p0 < o + d; p0 += BLOCK * kRowU) {
+      float x[kRowU];
+      gather_step<IDX, kRowU>(ind, v, p0, BLOCK, threadIdx.x, o + d, x);
+#pragma unroll
+      for (int t = 0; t < kRowU; ++t) acc = add_d(acc, (double)x[t]);
+    }
+    for (int s = 1; s < kWave; s <<= 1) acc = add_dd(acc, DD2{__shfl_xor(acc.hi, s), __shfl_xor(acc.lo, s)});
+    if (lane_id() == 0) {
+      part[0][threadIdx.x / kWave] = acc.hi;
+      part[1][threadIdx.x / kWave] = acc.lo;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      DD2 t{0.0, 0.0};
+      for (int w = 0; w < BLOCK / kWave; ++w) t = add_dd(t, DD2{part[0][w], part[1][w]});
+      out[u] = (OutT)t.hi;
+    }
+    __syncthreads();
+  }
+}
+
+size_t rowsum_scratch_bytes(int32_t n) { return ((size_t)std::max(n, 1) + 2) * sizeof(int); }
+
+// scratch: rowsum_scratch_bytes(n) bytes, or nullptr to use a stream-ordered
+// allocation for the call.
+template <typename OutT>
+static hipError_t rowsum_both(int32_t n, const int32_t* off, const int32_t* ind, const float* v, OutT* out,
+                              void* scratch, hipStream_t s) {
+  void* own = nullptr;
+  hipError_t err;
+  if (!scratch) {
+    if ((err = hipMallocAsync(&own, rowsum_scratch_bytes(n), s)) != hipSuccess) return err;
+    scratch = own;
+  }
+  int* counters = static_cast<int*>(scratch);
+  int* list = counters + 2;
+  const int cus = device_cu_count();
+  hipLaunchKernelGGL(k_fill<uint32_t>, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t*>(counters), (int64_t)2, 0u);
+  hipLaunchKernelGGL(k_collect_long, dim3(grid_for(n, 256, 8)), dim3(256), 0, s, n, off, list, counters);
+  if (ind) {
+    hipLaunchKernelGGL((k_rowsum_w<OutT, true>), dim3(grid_rows(n, 4, 64)), dim3(256), 0, s, n, off, ind, v, out);
+    hipLaunchKernelGGL((k_rowsum_w_long<OutT, true, 1024>), dim3(cus * 2), dim3(1024), 0, s, off, ind, v, out,
+                       list, counters);
+  } else {
+    hipLaunchKernelGGL((k_rowsum_w<OutT, false>), dim3(grid_rows(n, 4, 64)), dim3(256), 0, s, n, off, ind, v, out);
+    hipLaunchKernelGGL((k_rowsum_w_long<OutT, false, 1024>), dim3(cus * 2), dim3(1024), 0, s, off, ind, v, out,
+                       list, counters);
+  }
+  err = hipGetLastError();
+  if (own) {
+    hipError_t e2 = hipFreeAsync(own, s);
+    if (err == hipSuccess) err = e2;
+  }
+  return err;
+}
+
 hipError_t launch_rowsum_f32(int32_t n, const int32_t* off, const int32_t* ind, const float* v,
-                             float* work, hipStream_t s) {
+                             float* work, void* scratch, hipStream_t s) {
   ProfScope ps_("rowsum", s);
   if (n <= 0) return hipSuccess;
-  if (v == nullptr)
+  if (v == nullptr) {
     hipLaunchKernelGGL(k_rowsum_deg, dim3(grid_for(n, 256, 32)), dim3(256), 0, s, n, off, work);
-  else
-    hipLaunchKernelGGL(k_rowsum_w<float>, dim3(grid_for(n, 4, 64)), dim3(256), 0, s, n, off, ind, v, work);
-  return hipGetLastError();
+    return hipGetLastError();
+  }
+  return rowsum_both<float>(n, off, ind, v, work, scratch, s);
 }
 
 hipError_t launch_rowsum_f64(int32_t n, const int32_t* off, const int32_t* ind, const float* v,
-                             double* S, hipStream_t s) {
+                             double* S, void* scratch, hipStream_t s) {
   ProfScope ps_("rowsum", s);
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_rowsum_w<double>, dim3(grid_for(n, 4, 64)), dim3(256), 0, s, n, off, ind, v, S);
-  return hipGetLastError();
+  return rowsum_both<double>(n, off, ind, v, S, scratch, s);
 }
 
 hipError_t launch_vertex_weights(int32_t n, const int32_t* off, const float* edge_w, float* W,
-                                 hipStream_t s) {
+                                 void* scratch, hipStream_t s) {
   ProfScope ps_("vertex_weights", s);
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_rowsum_w<float>, dim3(grid_for(n, 4, 64)), dim3(256), 0, s, n, off,
-                     (const int32_t*)nullptr, edge_w, W);
-  return hipGetLastError();
+  return rowsum_both<float>(n, off, nullptr, edge_w, W, scratch, s);
 }
 
 // Jaccard weight: weight_j = Wi / (Ws - Wi) (jaccard.cpp:614-619).
@@ -180,7 +303,7 @@ __global__ void k_ec_scan(int32_t n, const int32_t* __restrict__ off, int32_t* _
 hipError_t launch_ec_scan(int32_t n, const int32_t* off, int32_t* dest, float* wj, hipStream_t s) {
   ProfScope ps_("ec_scan", s);
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_ec_scan, dim3(grid_for(n, 4, 64)), dim3(256), 0, s, n, off, dest, wj);
+  hipLaunchKernelGGL(k_ec_scan, dim3(grid_rows(n, 4, 64)), dim3(256), 0, s, n, off, dest, wj);
   return hipGetLastError();
 }
 
@@ -242,7 +365,7 @@ hipError_t launch_validate(int32_t n, int64_t m, const int32_t* off, const int32
 hipError_t launch_validate_rows(int32_t n, const int32_t* off, const int32_t* ind, int* flags,
                                 hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_validate_rows, dim3(grid_for(n, 4, 64)), dim3(256), 0, s, n, off, ind, flags);
+  hipLaunchKernelGGL(k_validate_rows, dim3(grid_rows(n, 4, 64)), dim3(256), 0, s, n, off, ind, flags);
   return hipGetLastError();
 }
 
@@ -266,7 +389,7 @@ hipError_t launch_symcheck(int32_t n, int64_t m, const int32_t* off, const int32
                            hipStream_t s) {
   (void)m;
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_symcheck, dim3(grid_for(n, 4, 64)), dim3(256), 0, s, n, off, ind, flag);
+  hipLaunchKernelGGL(k_symcheck, dim3(grid_rows(n, 4, 64)), dim3(256), 0, s, n, off, ind, flag);
   return hipGetLastError();
 }
 

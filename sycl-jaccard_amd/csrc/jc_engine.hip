@@ -9,18 +9,26 @@
 // Design for gfx950:
 //   * Each undirected pair {u, x} is computed once, by the endpoint with the
 //     larger (degree, id) -- its "owner" -- and written to both stored
-//     directions (the mirror position is found while streaming or by a wave
-//     search). Work is sum over pairs of min(d_u, d_x), half of the reference.
-//   * The owner's row is staged in LDS as an open-addressing hash table (key
-//     = column id; weighted: key + fp32 vertex weight in one 8-byte slot).
-//     The short row is streamed from HBM with coalesced 256-byte wave loads
-//     and each element is one LDS probe, instead of a dependent binary search
-//     through HBM.
-//   * Degree binning: rows with degree <= kSmallMax run one wave per row with a
-//     per-wave table; longer rows run one workgroup per (row, 512-entry piece)
-//     task from a degree-descending queue (persistent grid, atomic counter).
-//     Rows longer than the LDS chunk are split into chunks whose tables are
-//     built one after the other; per-item cursors/partials live in LDS.
+//     directions (the mirror position is found while streaming or by one
+//     search). Work is the sum over pairs of min(d_u, d_x): half of the
+//     reference's.
+//   * The owner's row is staged in LDS as a bucketized hash table: buckets
+//     of 8 keys (32 bytes, two ds_read_b128), at most two keys per bucket on
+//     average, weights in a parallel array. A probe is straight-line code;
+//     a full bucket (rare) sends the wave through a uniform slow path. The
+//     short row is streamed from HBM, several loads in flight per lane.
+//   * A wave is split into groups of G lanes, one short row per group, G and
+//     the unroll U chosen by the row's length class, so short rows do not
+//     leave lanes idle. Counts accumulate per lane (VALU) and are reduced
+//     once per row: scalar instructions stay off the inner loop.
+//   * Degree binning: rows with degree <= kSmallMax run one wave per row with
+//     a per-wave table; longer rows run one workgroup per (row, 512-entry
+//     piece) task from a degree-descending queue (persistent grid, atomic
+//     counter). A task sorts its owned entries into length classes, heaviest
+//     first, and its waves take class units from an LDS counter.
+//   * Rows longer than one LDS chunk are processed chunk by chunk; each short
+//     row resumes where the previous chunk stopped (no searches), its
+//     partial result kept in LDS.
 //   * Weighted sums are double-double. For fp32 addends whose bit span fits
 //     in 106 bits the sum is exact, so results do not depend on chunking,
 //     task split or lane order: plain, batched and pair paths agree bitwise.
@@ -67,68 +75,129 @@ __device__ __forceinline__ DD wave_sum_dd(DD v) {
 __device__ __forceinline__ void lds_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup"); }
 
 // ---------------------------------------------------------------------------
-// LDS hash tables.
+// Bucketized LDS hash table: 2^lb buckets of kBucket int32 slots. Slots 0..6
+// hold keys (-1 = empty); slot 7 is the spill marker, set when a key had to
+// move on to the next bucket. Weighted tables keep the fp32 weight of slot i
+// at val[i], so a probe reads keys and weights of one bucket with four
+// ds_read_b128 and selects with the same compares. Bucket = top bits of a
+// multiplicative hash.
+#ifndef JC_WVAL_SAMEPASS
+#define JC_WVAL_SAMEPASS 0
+#endif
+constexpr int kBucket = 8;
+constexpr int kBucketKeys = 7;
+constexpr int32_t kSpill = -2;
+
+struct Hit {
+  bool hit;
+  bool spill;
+  float v;
+};
+
 template <bool W>
-struct Table;
-
-template <>
-struct Table<false> {
-  using Slot = int32_t;
-  Slot* s;
-  int shift;
-  uint32_t mask;
-  __device__ __forceinline__ void clear(int i) const { s[i] = -1; }
-  __device__ __forceinline__ void insert(int32_t y, float) const {
-    uint32_t h = hslot(y, shift);
-    for (;;) {
-      const int32_t old = atomicCAS(&s[h], -1, y);
-      if (old == -1 || old == y) return;
-      h = (h + 1) & mask;
-    }
-  }
-  __device__ __forceinline__ bool probe(int32_t y, float&) const {
-    uint32_t h = hslot(y, shift);
-    for (;;) {
-      const int32_t k = s[h];
-      if (k == y) return true;
-      if (k < 0) return false;
-      h = (h + 1) & mask;
-    }
-  }
-};
-
-template <>
-struct Table<true> {
-  using Slot = int2;  // .x key, .y fp32 weight bits
-  Slot* s;
-  int shift;
-  uint32_t mask;
-  __device__ __forceinline__ void clear(int i) const { s[i] = make_int2(-1, 0); }
+struct Table {
+  int32_t* key;
+  float* val;
+  int shift;        // bucket = (key * C) >> shift
+  uint32_t bmask;   // buckets - 1
+  __device__ __forceinline__ uint32_t bucket(int32_t y) const { return ((uint32_t)y * 0x9E3779B1u) >> shift; }
+  __device__ __forceinline__ uint32_t next(uint32_t b) const { return (b + 1) & bmask; }
+  __device__ __forceinline__ void clear(int i) const { key[i] = -1; }
   __device__ __forceinline__ void insert(int32_t y, float v) const {
-    uint32_t h = hslot(y, shift);
+    uint32_t b = bucket(y);
     for (;;) {
-      const int32_t old = atomicCAS(&s[h].x, -1, y);
-      if (old == -1) {
-        s[h].y = __float_as_int(v);
-        return;
+      int32_t* kb = key + (b * kBucket);
+      for (int s = 0; s < kBucketKeys; ++s) {
+        const int32_t old = atomicCAS(&kb[s], -1, y);
+        if (old == -1) {
+          if constexpr (W) val[b * kBucket + s] = v;
+          return;
+        }
+        if (old == y) return;
       }
-      if (old == y) return;
-      h = (h + 1) & mask;
+      kb[kBucketKeys] = kSpill;
+      b = next(b);
     }
   }
-  __device__ __forceinline__ bool probe(int32_t y, float& v) const {
-    uint32_t h = hslot(y, shift);
-    for (;;) {
-      const int2 k = s[h];
-      if (k.x == y) {
-        v = __int_as_float(k.y);
-        return true;
-      }
-      if (k.x < 0) return false;
-      h = (h + 1) & mask;
+  __device__ __forceinline__ Hit look(uint32_t b, int32_t y) const {
+    const int4* kp = reinterpret_cast<const int4*>(key + b * kBucket);
+    const int4 k0 = kp[0];
+    const int4 k1 = kp[1];
+    const bool e0 = k0.x == y, e1 = k0.y == y, e2 = k0.z == y, e3 = k0.w == y;
+    const bool e4 = k1.x == y, e5 = k1.y == y, e6 = k1.z == y;
+    Hit h;
+    h.hit = e0 | e1 | e2 | e3 | e4 | e5 | e6;
+    h.spill = k1.w == kSpill;
+    h.v = 0.0f;
+#if JC_WVAL_SAMEPASS
+    if constexpr (W) {
+      const float4* vp = reinterpret_cast<const float4*>(val + b * kBucket);
+      const float4 v0 = vp[0];
+      const float4 v1 = vp[1];
+      float v = e6 ? v1.z : 0.0f;
+      v = e5 ? v1.y : v;
+      v = e4 ? v1.x : v;
+      v = e3 ? v0.w : v;
+      v = e2 ? v0.z : v;
+      v = e1 ? v0.y : v;
+      v = e0 ? v0.x : v;
+      h.v = v;
     }
+#else
+    if constexpr (W) {
+      int slot = e6 ? 6 : 0;
+      slot = e5 ? 5 : slot;
+      slot = e4 ? 4 : slot;
+      slot = e3 ? 3 : slot;
+      slot = e2 ? 2 : slot;
+      slot = e1 ? 1 : slot;
+      const float v = val[b * kBucket + slot];
+      h.v = h.hit ? v : 0.0f;
+    }
+#endif
+    return h;
   }
 };
+
+// Average keys per bucket (of kBucketKeys key slots) at full chunk size:
+// 3 for key-only tables, 2 for weighted ones (whose LDS holds weights too;
+// denser buckets spill more and cost more there than fewer chunks save).
+#ifndef JC_LAMBDA_U
+#define JC_LAMBDA_U 3
+#endif
+#ifndef JC_LAMBDA_W
+#define JC_LAMBDA_W 2
+#endif
+template <bool W>
+constexpr int kLambda = W ? JC_LAMBDA_W : JC_LAMBDA_U;
+
+// log2 of the bucket count for a chunk of `len` keys: at most kLambda keys
+// per bucket on average, at least 4 buckets.
+template <bool W>
+__device__ __forceinline__ int table_log_buckets(int len) {
+  return max(ceil_log2((len + kLambda<W> - 1) / kLambda<W>), 2);
+}
+
+// Inserts keys ind[cs .. cs+len) (weighted: with vw[key]) into the table,
+// `stride` threads cooperating; four keys per thread are loaded before any
+// insert so the loads overlap.
+template <bool W>
+__device__ __forceinline__ void build_table(const Table<W>& tab, const int32_t* __restrict__ ind,
+                                            const float* __restrict__ vw, int cs, int len, int tid, int stride) {
+  for (int i0 = tid; i0 < len; i0 += 4 * stride) {
+    int32_t y[4];
+    float v[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int t = 0; t < 4; ++t) y[t] = ind[cs + min(i0 + t * stride, len - 1)];
+    if constexpr (W) {
+#pragma unroll
+      for (int t = 0; t < 4; ++t) v[t] = vw[y[t]];
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+      if (i0 + t * stride < len) tab.insert(y[t], v[t]);
+  }
+}
 
 // ---------------------------------------------------------------------------
 // Epilogue: write the result of pair (u, x) at position k and, if q >= 0, at
@@ -166,46 +235,263 @@ __device__ __forceinline__ void emit(const EngineArgs& a, int32_t u, int du, int
   }
 }
 
-// Streams the whole short row [ox, ox+dx), probes elements inside [lo, hi],
-// and locates u inside it (the mirror position). Wave-uniform results.
+// ---------------------------------------------------------------------------
+// Group primitives. A wave is split into 64/G groups of G lanes; each group
+// works on its own item. Ballots are masked to the group, loops are
+// group-uniform (groups of one wave may run different trip counts).
+template <int G>
+__device__ __forceinline__ uint64_t group_mask() {
+  if constexpr (G == kWave)
+    return ~0ull;
+  else
+    return ((1ull << G) - 1) << (lane_id() & ~(G - 1));
+}
+
+// lower_bound within one group: smallest i in [lo, hi) with a[i] >= key.
+template <int G>
+__device__ __forceinline__ int grp_lower_bound(const int32_t* __restrict__ a, int lo, int hi, int32_t key) {
+  const uint64_t gm = group_mask<G>();
+  const int gl = lane_id() & (G - 1);
+  while (hi - lo > G) {
+    const int stride = (hi - lo + G - 1) / G;
+    const int idx = lo + gl * stride;
+    const bool lt = idx < hi && a[idx] < key;
+    const int c = popc64(ballot(lt) & gm);
+    if (c == 0) return lo;
+    const int nlo = lo + (c - 1) * stride + 1;
+    const int nhi = min(hi, lo + c * stride);
+    lo = nlo;
+    hi = nhi;
+  }
+  const int idx = lo + gl;
+  const bool lt = idx < hi && a[idx] < key;
+  return lo + popc64(ballot(lt) & gm);
+}
+
+template <int G>
+__device__ __forceinline__ int grp_sum_i(int v) {
+#pragma unroll
+  for (int s = 1; s < G; s <<= 1) v += __shfl_xor(v, s);
+  return v;
+}
+
+template <int G>
+__device__ __forceinline__ DD grp_sum_dd(DD v) {
+#pragma unroll
+  for (int s = 1; s < G; s <<= 1) {
+    DD o{__shfl_xor(v.hi, s), __shfl_xor(v.lo, s)};
+    v = dd_add(v, o);
+  }
+  return v;
+}
+
+// Per-chunk context of an owner row u.
 template <bool W>
-__device__ __forceinline__ void full_pass(const int32_t* __restrict__ ind, int ox, int dx, int32_t u,
-                                          int32_t lo, int32_t hi, const Table<W>& tab,
-                                          uint32_t& cnt, DD& acc, int& below, bool& found) {
+struct Chunk {
+  Table<W> tab;
+  int32_t u;
+  int du;
+  int32_t lo, hi;   // id range of the chunk's columns
+  int c, nch;       // chunk index / count
+  int64_t kbase;    // position of task entry j is kbase + j
+  // multi-chunk scratch, indexed by item: resume position and partial result
+  int* cursor;
+  uint32_t* pcnt;
+  DD* pacc;
+};
+
+// Probes U elements per lane: per-lane hit count and weight sum.
+template <bool W, int U>
+__device__ __forceinline__ void probe_u(const Chunk<W>& cx, const int32_t (&y)[U], int& cnt, DD& acc) {
+  uint32_t bk[U];
+  Hit h[U];
+  bool anyneed = false;
+#pragma unroll
+  for (int t = 0; t < U; ++t) {
+    bk[t] = cx.tab.bucket(y[t]);
+    h[t] = cx.tab.look(bk[t], y[t]);
+    anyneed |= h[t].spill && !h[t].hit;
+  }
+  if (ballot(anyneed)) {
+#pragma unroll
+    for (int t = 0; t < U; ++t) {
+      while (h[t].spill && !h[t].hit) {
+        bk[t] = cx.tab.next(bk[t]);
+        h[t] = cx.tab.look(bk[t], y[t]);
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < U; ++t) {
+    cnt += h[t].hit ? 1 : 0;
+    if constexpr (W) acc = dd_add1(acc, (double)h[t].v);
+  }
+}
+
+// One unit of work: each group of G lanes handles one owned short row x
+// (valid), U loads per lane in flight.
+//   MULTI = false (the owner row fits one chunk): stream all of N(x), probe,
+//     count the entries below u (mirror position) and emit.
+//   MULTI = true: from the row's resume position, stream entries up to the
+//     chunk's id bound, store the partial and the resume position; on the
+//     last chunk stream to the end, add the stored partial, locate u by one
+//     group search and emit.
+template <int MODE, bool W, int G, int U, bool MULTI>
+__device__ __forceinline__ void unit(const EngineArgs& a, const Chunk<W>& cx, bool valid, int32_t x,
+                                     int ox, int dx, int j, int item) {
+  const int gl = lane_id() & (G - 1);
+  const uint64_t gm = group_mask<G>();
   const int end = ox + dx;
-  for (int p0 = ox; p0 < end; p0 += kWave) {
-    const int p = p0 + lane_id();
-    const bool act = p < end;
-    const int32_t y = act ? ind[p] : INT32_MAX;
-    below += popc64(ballot(act && y < u));
-    found |= ballot(act && y == u) != 0;
-    bool hit = false;
-    float v = 0.0f;
-    if (act && y >= lo && y <= hi) hit = tab.probe(y, v);
-    cnt += popc64(ballot(hit));
-    if constexpr (W) {
-      if (hit) acc = dd_add1(acc, (double)v);
+  const bool last = !MULTI || cx.c == cx.nch - 1;
+  int s = ox;
+  if constexpr (MULTI) s = (valid && cx.c > 0) ? cx.cursor[item] : ox;
+  const int e = valid ? end : s;
+  // rows streamed in one group pass count the entries below u on the way;
+  // longer rows locate u with one group search
+  constexpr bool kCountMirror = !MULTI && G * U <= 64;
+  int cnt = 0, below = 0, found = 0;
+  DD acc{0.0, 0.0};
+  int stop_at = e;
+  for (int p0 = s; p0 < e; p0 += G * U) {
+    // unconditional loads (clamped index) so all U are in flight together
+    int32_t y[U];
+#pragma unroll
+    for (int t = 0; t < U; ++t) y[t] = a.ind[min(p0 + t * G + gl, e - 1)];
+#pragma unroll
+    for (int t = 0; t < U; ++t) {
+      const int p = p0 + t * G + gl;
+      if (p >= e) y[t] = INT32_MAX;
+      if constexpr (kCountMirror) {
+        below += y[t] < cx.u ? 1 : 0;
+        found |= y[t] == cx.u ? 1 : 0;
+      }
+    }
+    probe_u<W, U>(cx, y, cnt, acc);
+    if constexpr (MULTI) {
+      if (!last) {
+        // entries beyond the chunk's bound belong to later chunks: stop at the first
+        int first = -1;
+#pragma unroll
+        for (int t = 0; t < U; ++t) {
+          const int p = p0 + t * G + gl;
+          const uint64_t bm = ballot(p < e && y[t] > cx.hi) & gm;
+          if (first < 0 && bm) first = p0 + t * G + (__ffsll((unsigned long long)bm) - 1 - (lane_id() & ~(G - 1)));
+        }
+        if (first >= 0) {
+          stop_at = first;
+          break;
+        }
+      }
+    }
+  }
+  cnt = grp_sum_i<G>(cnt);
+  if constexpr (W) acc = grp_sum_dd<G>(acc);
+  if constexpr (!MULTI) {
+    int64_t q = -1;
+    if constexpr (kCountMirror) {
+      below = grp_sum_i<G>(below);
+      found = grp_sum_i<G>(found);
+      if (found) q = (int64_t)ox + below;
+    } else {
+      int pos = end;
+      if (valid) pos = grp_lower_bound<G>(a.ind, ox, end, cx.u);
+      if (valid && pos < end && a.ind[pos] == cx.u) q = pos;
+    }
+    if (valid && gl == 0) emit<MODE, W>(a, cx.u, cx.du, x, dx, cx.kbase + j, q, (uint32_t)cnt, acc);
+  } else {
+    if (!last) {
+      if (valid && gl == 0) {
+        cx.cursor[item] = stop_at;
+        if constexpr (W)
+          cx.pacc[item] = cx.c == 0 ? acc : dd_add(cx.pacc[item], acc);
+        else
+          cx.pcnt[item] = cx.c == 0 ? (uint32_t)cnt : cx.pcnt[item] + (uint32_t)cnt;
+      }
+    } else {
+      int q = end;
+      if (valid) q = grp_lower_bound<G>(a.ind, ox, end, cx.u);
+      if (valid && gl == 0) {
+        uint32_t tot = (uint32_t)cnt;
+        if constexpr (W)
+          acc = dd_add(cx.pacc[item], acc);
+        else
+          tot += cx.pcnt[item];
+        const bool f = q < end && a.ind[q] == cx.u;
+        emit<MODE, W>(a, cx.u, cx.du, x, dx, cx.kbase + j, f ? (int64_t)q : -1, tot, acc);
+      }
     }
   }
 }
 
-// Streams positions [s, e) of a short row and probes every element.
-template <bool W>
-__device__ __forceinline__ void range_pass(const int32_t* __restrict__ ind, int s, int e, int32_t lo,
-                                           int32_t hi, const Table<W>& tab, uint32_t& cnt, DD& acc) {
-  for (int p0 = s; p0 < e; p0 += kWave) {
-    const int p = p0 + lane_id();
-    bool hit = false;
-    float v = 0.0f;
-    if (p < e) {
-      const int32_t y = ind[p];
-      if (y >= lo && y <= hi) hit = tab.probe(y, v);
+// Length classes of a short row (entries streamed per chunk):
+//   0: <= 8     G 8,  U 1      3: <= 64   G 16, U 4
+//   1: <= 16    G 16, U 1      4: <= 256  G 32, U 4
+//   2: <= 32    G 16, U 2      5: > 256   G 64, U 4
+#ifndef JC_U4
+#define JC_U4 4
+#endif
+#ifndef JC_U5
+#define JC_U5 4
+#endif
+constexpr int kClasses = 6;
+__host__ __device__ __forceinline__ int len_class(int len) {
+  return len <= 8 ? 0 : len <= 16 ? 1 : len <= 32 ? 2 : len <= 64 ? 3 : len <= 256 ? 4 : 5;
+}
+__host__ __device__ __forceinline__ constexpr int class_g(int c) {
+  return c == 0 ? 8 : c <= 3 ? 16 : c == 4 ? 32 : 64;
+}
+
+template <int MODE, bool W, bool MULTI, int C>
+__device__ __forceinline__ void unit_class(const EngineArgs& a, const Chunk<W>& cx, bool valid, int32_t x,
+                                           int ox, int dx, int j, int item) {
+  constexpr int G = class_g(C);
+  constexpr int U = C <= 1 ? 1 : C == 2 ? 2 : C == 3 ? 4 : C == 4 ? JC_U4 : JC_U5;
+  unit<MODE, W, G, U, MULTI>(a, cx, valid, x, ox, dx, j, item);
+}
+
+// Items of one class held in lanes (x, ox, dx, j per lane; mask = owned lanes
+// of the class): processed 64/G at a time, group g taking the g-th item.
+template <int MODE, bool W, int C>
+__device__ __forceinline__ void run_class_regs(const EngineArgs& a, const Chunk<W>& cx, uint64_t mask,
+                                               int32_t x, int ox, int dx, int j) {
+  constexpr int G = class_g(C);
+  constexpr int NG = kWave / G;
+  const int g = lane_id() / G;
+  while (mask) {
+    int src = -1;
+#pragma unroll
+    for (int i = 0; i < NG; ++i) {
+      if (mask) {
+        const int b = __ffsll((unsigned long long)mask) - 1;
+        if (i == g) src = b;
+        mask &= mask - 1;
+      }
     }
-    cnt += popc64(ballot(hit));
-    if constexpr (W) {
-      if (hit) acc = dd_add1(acc, (double)v);
-    }
+    const int s = src < 0 ? 0 : src;
+    const int32_t xx = __shfl(x, s);
+    const int oxx = __shfl(ox, s);
+    const int dxx = __shfl(dx, s);
+    const int jj = __shfl(j, s);
+    unit_class<MODE, W, false, C>(a, cx, src >= 0, xx, oxx, dxx, jj, 0);
   }
+}
+
+// Candidate neighbours in lanes: split the owned ones by short-row length and
+// run each class (single-chunk owner rows).
+template <int MODE, bool W>
+__device__ __forceinline__ void run_batch_regs(const EngineArgs& a, const Chunk<W>& cx, bool cand, int32_t x,
+                                               int ox, int dx, int j) {
+  const bool own = cand && owns(cx.u, cx.du, x, dx);
+  const int c = len_class(dx);
+  uint64_t m[kClasses];
+#pragma unroll
+  for (int k = 0; k < kClasses; ++k) m[k] = ballot(own && c == k);
+  if (m[5]) run_class_regs<MODE, W, 5>(a, cx, m[5], x, ox, dx, j);
+  if (m[4]) run_class_regs<MODE, W, 4>(a, cx, m[4], x, ox, dx, j);
+  if (m[3]) run_class_regs<MODE, W, 3>(a, cx, m[3], x, ox, dx, j);
+  if (m[2]) run_class_regs<MODE, W, 2>(a, cx, m[2], x, ox, dx, j);
+  if (m[1]) run_class_regs<MODE, W, 1>(a, cx, m[1], x, ox, dx, j);
+  if (m[0]) run_class_regs<MODE, W, 0>(a, cx, m[0], x, ox, dx, j);
 }
 
 // ---------------------------------------------------------------------------
@@ -213,9 +499,9 @@ __device__ __forceinline__ void range_pass(const int32_t* __restrict__ ind, int 
 template <int MODE, bool W, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_owner_small(EngineArgs a) {
   constexpr int NW = BLOCK / kWave;
-  constexpr int PW = 2 * kSmallMax;
-  using Slot = typename Table<W>::Slot;
-  __shared__ Slot tab_all[NW][PW];
+  constexpr int SLOTS = kBucket << 6;  // 64 buckets hold a row of up to 128 keys at <= 2 per bucket
+  __shared__ __align__(16) int32_t key_all[NW][SLOTS];
+  __shared__ __align__(16) float val_all[W ? NW : 1][W ? SLOTS : 1];
   const int w = threadIdx.x / kWave;
   const int lane = lane_id();
   const int nsmall = *a.num_small;
@@ -224,7 +510,6 @@ __global__ __launch_bounds__(BLOCK) void k_owner_small(EngineArgs a) {
     const int32_t u = a.small_rows[r];
     const int ou = a.off[u];
     const int du = a.off[u + 1] - ou;
-    // owned neighbours of u (du <= 128: two lane groups)
     int32_t x0 = 0, x1 = 0;
     int dx0 = 0, dx1 = 0, ox0 = 0, ox1 = 0;
     if (lane < du) {
@@ -237,73 +522,106 @@ __global__ __launch_bounds__(BLOCK) void k_owner_small(EngineArgs a) {
       ox1 = a.off[x1];
       dx1 = a.off[x1 + 1] - ox1;
     }
-    uint64_t own0 = ballot(lane < du && owns(u, du, x0, dx0));
-    uint64_t own1 = ballot(lane + kWave < du && owns(u, du, x1, dx1));
-    if ((own0 | own1) == 0) continue;
-    const int logP = max(ceil_log2(2 * du), 5);
-    const Table<W> tab{tab_all[w], 32 - logP, (uint32_t)((1 << logP) - 1)};
-    for (int i = lane; i < (1 << logP); i += kWave) tab.clear(i);
+    const bool c0 = lane < du, c1 = lane + kWave < du;
+    if ((ballot(c0 && owns(u, du, x0, dx0)) | ballot(c1 && owns(u, du, x1, dx1))) == 0) continue;
+    const int lb = table_log_buckets<W>(du);
+    Chunk<W> cx;
+    cx.tab = Table<W>{key_all[w], W ? val_all[W ? w : 0] : nullptr, 32 - lb, (uint32_t)((1 << lb) - 1)};
+    for (int i = lane; i < (kBucket << lb); i += kWave) cx.tab.clear(i);
     lds_fence();
-    for (int i = lane; i < du; i += kWave) {
-      const int32_t y = a.ind[ou + i];
-      tab.insert(y, W ? a.vw[y] : 0.0f);
-    }
+    build_table<W>(cx.tab, a.ind, a.vw, ou, du, lane, kWave);
     lds_fence();
-    const int32_t lo = a.ind[ou], hi = a.ind[ou + du - 1];
-    for (int g = 0; g < 2; ++g) {
-      uint64_t own = g == 0 ? own0 : own1;
-      while (own) {
-        const int l = __ffsll((unsigned long long)own) - 1;
-        own &= own - 1;
-        const int32_t x = __shfl(g == 0 ? x0 : x1, l);
-        const int ox = __shfl(g == 0 ? ox0 : ox1, l);
-        const int dx = __shfl(g == 0 ? dx0 : dx1, l);
-        uint32_t cnt = 0;
-        DD acc{0.0, 0.0};
-        int below = 0;
-        bool found = false;
-        full_pass<W>(a.ind, ox, dx, u, lo, hi, tab, cnt, acc, below, found);
-        if constexpr (W) acc = wave_sum_dd(acc);
-        if (lane == 0)
-          emit<MODE, W>(a, u, du, x, dx, (int64_t)ou + g * kWave + l, found ? (int64_t)ox + below : -1,
-                        cnt, acc);
-      }
-    }
+    cx.u = u;
+    cx.du = du;
+    cx.lo = a.ind[ou];
+    cx.hi = a.ind[ou + du - 1];
+    cx.c = 0;
+    cx.nch = 1;
+    cx.kbase = ou;
+    cx.cursor = nullptr;
+    cx.pcnt = nullptr;
+    cx.pacc = nullptr;
+    run_batch_regs<MODE, W>(a, cx, c0, x0, ox0, dx0, lane);
+    run_batch_regs<MODE, W>(a, cx, c1, x1, ox1, dx1, lane + kWave);
     lds_fence();  // all probes of this row done before the next row clears the table
   }
 }
 
 // ---------------------------------------------------------------------------
 // Rows with degree > kSmallMax: one workgroup per (row, piece) task.
+// LDS of one task workgroup (one per CU, the whole 160 KiB): what the item
+// list and per-item state leave is the hash table. Measured on RMAT-22 this
+// beats two 80 KiB workgroups per CU: halving the chunk count of hub rows
+// outweighs the idle time at chunk barriers.
+#ifndef JC_BLOCK_LDS_KIB
+#define JC_BLOCK_LDS_KIB 160
+#endif
+#ifndef JC_BLOCK
+#define JC_BLOCK 1024
+#endif
+constexpr size_t kBlockLds = JC_BLOCK_LDS_KIB * 1024 - 512;  // minus the kernel's static shared variables
+
 template <bool W>
 struct BlockCfg {
-  static constexpr int CH = W ? kChunkW : kChunkU;
-  static constexpr int PMAX = 2 * CH;
-  using Slot = typename Table<W>::Slot;
-  static constexpr size_t table_bytes = (size_t)PMAX * sizeof(Slot);
-  static constexpr size_t acc_bytes = (size_t)kTaskSpan * (W ? sizeof(DD) : sizeof(uint32_t));
+  static constexpr size_t item_bytes = (size_t)kTaskSpan * (3 * sizeof(int32_t) + sizeof(uint16_t));
   static constexpr size_t cursor_bytes = (size_t)kTaskSpan * sizeof(int);
-  static constexpr size_t lds_bytes = table_bytes + acc_bytes + cursor_bytes;
+  static constexpr size_t part_bytes = (size_t)kTaskSpan * (W ? sizeof(DD) : sizeof(uint32_t));
+  static constexpr size_t bucket_bytes = kBucket * (sizeof(int32_t) + (W ? sizeof(float) : 0));
+  static constexpr int NB_FIT = (int)((kBlockLds - item_bytes - cursor_bytes - part_bytes) / bucket_bytes);
+  static constexpr int NB = 1 << (31 - __builtin_clz(NB_FIT));  // power of two
+  static constexpr int CH = kLambda<W> * NB;  // keys per chunk
+  static constexpr size_t key_bytes = (size_t)NB * kBucket * sizeof(int32_t);
+  static constexpr size_t val_bytes = W ? key_bytes : 0;
+  static constexpr size_t lds_bytes = key_bytes + val_bytes + item_bytes + cursor_bytes + part_bytes;
 };
 
-// Positions a wave takes from the task per grab: small grabs keep the waves of
-// a workgroup balanced when neighbour degrees vary widely.
-constexpr int kGrab = 16;
+template <int MODE, bool W, bool MULTI>
+__device__ __forceinline__ void run_unit_lds(const EngineArgs& a, const Chunk<W>& cx, int cls, int first,
+                                             int cend, const int32_t* ix, const int32_t* iox,
+                                             const int32_t* idx_, const uint16_t* ij) {
+  const int G = class_g(cls);
+  const int item = first + lane_id() / G;
+  const bool valid = item < cend;
+  const int it = valid ? item : first;
+  const int32_t x = ix[it];
+  const int ox = iox[it];
+  const int dx = idx_[it];
+  const int j = ij[it];
+  switch (cls) {
+    case 5: unit_class<MODE, W, MULTI, 5>(a, cx, valid, x, ox, dx, j, it); break;
+    case 4: unit_class<MODE, W, MULTI, 4>(a, cx, valid, x, ox, dx, j, it); break;
+    case 3: unit_class<MODE, W, MULTI, 3>(a, cx, valid, x, ox, dx, j, it); break;
+    case 2: unit_class<MODE, W, MULTI, 2>(a, cx, valid, x, ox, dx, j, it); break;
+    case 1: unit_class<MODE, W, MULTI, 1>(a, cx, valid, x, ox, dx, j, it); break;
+    default: unit_class<MODE, W, MULTI, 0>(a, cx, valid, x, ox, dx, j, it); break;
+  }
+}
 
 template <int MODE, bool W, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
+  static_assert(BLOCK >= kTaskSpan, "one thread per task entry during compaction");
   using Cfg = BlockCfg<W>;
-  using Slot = typename Cfg::Slot;
   constexpr int CH = Cfg::CH;
+  constexpr int NW = BLOCK / kWave;
   extern __shared__ __align__(16) unsigned char smem[];
-  Slot* tslots = reinterpret_cast<Slot*>(smem);
-  DD* acc_dd = reinterpret_cast<DD*>(smem + Cfg::table_bytes);
-  uint32_t* acc_cnt = reinterpret_cast<uint32_t*>(smem + Cfg::table_bytes);
-  int* cursor = reinterpret_cast<int*>(smem + Cfg::table_bytes + Cfg::acc_bytes);
+  int32_t* tkey = reinterpret_cast<int32_t*>(smem);
+  float* tval = reinterpret_cast<float*>(smem + Cfg::key_bytes);
+  int32_t* ix = reinterpret_cast<int32_t*>(smem + Cfg::key_bytes + Cfg::val_bytes);
+  int32_t* iox = ix + kTaskSpan;
+  int32_t* idx_ = iox + kTaskSpan;
+  uint16_t* ij = reinterpret_cast<uint16_t*>(idx_ + kTaskSpan);
+  int* cursor = reinterpret_cast<int*>(smem + Cfg::key_bytes + Cfg::val_bytes + Cfg::item_bytes);
+  unsigned char* partp = smem + Cfg::key_bytes + Cfg::val_bytes + Cfg::item_bytes + Cfg::cursor_bytes;
+  uint32_t* pcnt = reinterpret_cast<uint32_t*>(partp);
+  DD* pacc = reinterpret_cast<DD*>(partp);
   __shared__ int s_task;
   __shared__ int s_next;
+  __shared__ int s_wcnt[kClasses][NW];
+  __shared__ int s_cbase[kClasses + 1];   // item ranges per class, heaviest first
+  __shared__ int s_ubase[kClasses + 1];   // unit ranges per class
   const int tid = threadIdx.x;
   const int lane = lane_id();
+  const int w = tid / kWave;
   const int ntasks = *a.num_tasks;
   for (;;) {
     __syncthreads();
@@ -318,86 +636,84 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
     const int ka = ou + tk.y * kTaskSpan;
     const int span = min(kTaskSpan, ou + du - ka);
     const int nch = (du + CH - 1) / CH;
+
+    // owned entries of the task, sorted into length classes (heaviest first)
+    int32_t x = 0;
+    int ox = 0, dx = 0, cls = -1;
+    if (tid < span) {
+      x = a.ind[ka + tid];
+      ox = a.off[x];
+      dx = a.off[x + 1] - ox;
+      if (owns(u, du, x, dx)) cls = len_class((dx + nch - 1) / nch);
+    }
+    int rank = 0;
+#pragma unroll
+    for (int k = 0; k < kClasses; ++k) {
+      const uint64_t bm = ballot(cls == k);
+      if (cls == k) rank = popc64(bm & ((1ull << lane) - 1));
+      if (lane == 0) s_wcnt[k][w] = popc64(bm);
+    }
+    __syncthreads();
+    if (tid == 0) {
+      int ib = 0, ub = 0;
+      for (int k = kClasses - 1; k >= 0; --k) {
+        int c = 0;
+        for (int v = 0; v < NW; ++v) c += s_wcnt[k][v];
+        s_cbase[kClasses - 1 - k] = ib;
+        s_ubase[kClasses - 1 - k] = ub;
+        ib += c;
+        ub += (c + (kWave / class_g(k)) - 1) / (kWave / class_g(k));
+      }
+      s_cbase[kClasses] = ib;
+      s_ubase[kClasses] = ub;
+    }
+    __syncthreads();
+    if (cls >= 0) {
+      int at = s_cbase[kClasses - 1 - cls] + rank;
+      for (int v = 0; v < w; ++v) at += s_wcnt[cls][v];
+      ix[at] = x;
+      iox[at] = ox;
+      idx_[at] = dx;
+      ij[at] = (uint16_t)tid;
+    }
+    const int nunits = s_ubase[kClasses];
     for (int c = 0; c < nch; ++c) {
       const int cs = ou + c * CH;
       const int ce = min(cs + CH, ou + du);
       const int clen = ce - cs;
-      const int logP = max(ceil_log2(2 * clen), 5);
-      const Table<W> tab{tslots, 32 - logP, (uint32_t)((1 << logP) - 1)};
-      for (int i = tid; i < (1 << logP); i += BLOCK) tab.clear(i);
+      const int lb = table_log_buckets<W>(clen);
+      Chunk<W> cx;
+      cx.tab = Table<W>{tkey, tval, 32 - lb, (uint32_t)((1 << lb) - 1)};
+      for (int i = tid; i < (kBucket << lb); i += BLOCK) cx.tab.clear(i);
       if (tid == 0) s_next = 0;
       __syncthreads();
-      for (int i = tid; i < clen; i += BLOCK) {
-        const int32_t y = a.ind[cs + i];
-        tab.insert(y, W ? a.vw[y] : 0.0f);
-      }
+      build_table<W>(cx.tab, a.ind, a.vw, cs, clen, tid, BLOCK);
       __syncthreads();
-      const int32_t lo = a.ind[cs], hi = a.ind[ce - 1];
-      const bool last = c == nch - 1;
+      cx.u = u;
+      cx.du = du;
+      cx.lo = a.ind[cs];
+      cx.hi = a.ind[ce - 1];
+      cx.c = c;
+      cx.nch = nch;
+      cx.kbase = ka;
+      cx.cursor = cursor;
+      cx.pcnt = pcnt;
+      cx.pacc = pacc;
       for (;;) {
-        int base = 0;
-        if (lane == 0) base = atomicAdd(&s_next, kGrab);
-        base = __shfl(base, 0);
-        if (base >= span) break;
-        const int j = base + lane;
-        int32_t x = 0;
-        int ox = 0, dx = 0;
-        if (lane < kGrab && j < span) {
-          x = a.ind[ka + j];
-          ox = a.off[x];
-          dx = a.off[x + 1] - ox;
-        }
-        uint64_t own = ballot(lane < kGrab && j < span && owns(u, du, x, dx));
-        while (own) {
-          const int l = __ffsll((unsigned long long)own) - 1;
-          own &= own - 1;
-          const int jj = base + l;
-          const int32_t xx = __shfl(x, l);
-          const int oxx = __shfl(ox, l);
-          const int dxx = __shfl(dx, l);
-          uint32_t cnt = 0;
-          DD acc{0.0, 0.0};
-          if (nch == 1) {
-            int below = 0;
-            bool found = false;
-            full_pass<W>(a.ind, oxx, dxx, u, lo, hi, tab, cnt, acc, below, found);
-            if constexpr (W) acc = wave_sum_dd(acc);
-            if (lane == 0)
-              emit<MODE, W>(a, u, du, xx, dxx, (int64_t)ka + jj, found ? (int64_t)oxx + below : -1,
-                            cnt, acc);
-            continue;
-          }
-          const int end = oxx + dxx;
-          const int s = c == 0 ? oxx : cursor[jj];
-          int e = s;
-          if (s < end) {
-            if (last)
-              e = end;
-            else if (a.ind[s] <= hi)
-              e = wave_lower_bound(a.ind, s, end, hi + 1);
-          }
-          range_pass<W>(a.ind, s, e, lo, hi, tab, cnt, acc);
-          if constexpr (W) acc = wave_sum_dd(acc);
-          if (!last) {
-            if (lane == 0) {
-              cursor[jj] = e;
-              if constexpr (W) {
-                acc_dd[jj] = c == 0 ? acc : dd_add(acc_dd[jj], acc);
-              } else {
-                acc_cnt[jj] = c == 0 ? cnt : acc_cnt[jj] + cnt;
-              }
-            }
-          } else {
-            if constexpr (W) {
-              acc = dd_add(acc_dd[jj], acc);
-            } else {
-              cnt += acc_cnt[jj];
-            }
-            const int q = wave_lower_bound(a.ind, oxx, end, u);
-            const bool found = q < end && a.ind[q] == u;
-            if (lane == 0) emit<MODE, W>(a, u, du, xx, dxx, (int64_t)ka + jj, found ? (int64_t)q : -1, cnt, acc);
-          }
-        }
+        int un = 0;
+        if (lane == 0) un = atomicAdd(&s_next, 1);
+        un = __shfl(un, 0);
+        if (un >= nunits) break;
+        int k = 0;
+        while (un >= s_ubase[k + 1]) ++k;  // class slot k (0 = heaviest)
+        const int cls_k = kClasses - 1 - k;
+        const int per = kWave / class_g(cls_k);
+        const int first = s_cbase[k] + (un - s_ubase[k]) * per;
+        const int cend = s_cbase[k + 1];
+        if (nch == 1)
+          run_unit_lds<MODE, W, false>(a, cx, cls_k, first, cend, ix, iox, idx_, ij);
+        else
+          run_unit_lds<MODE, W, true>(a, cx, cls_k, first, cend, ix, iox, idx_, ij);
       }
       __syncthreads();
     }
@@ -603,7 +919,7 @@ size_t engine_workspace_bytes(int32_t n, int64_t m) {
 template <int MODE, bool W>
 static hipError_t run_engine_t(const EngineArgs& base, void* ws, bool fixup, hipStream_t stream) {
   constexpr int SB = 256;   // small-row kernel block
-  constexpr int BB = 512;   // block kernel block
+  constexpr int BB = JC_BLOCK;  // block kernel block
   constexpr int PB = 256;   // schedule kernels
   EngineArgs a = base;
   unsigned char* p = static_cast<unsigned char*>(ws);

@@ -16,9 +16,13 @@ enum Mode : int {
 
 // Rows up to this degree are handled one wave per row; longer rows by a
 // workgroup per task. Tasks cover at most kTaskSpan consecutive entries of a
-// row.
+// row (sized so item list, per-item state and a 128 KiB hash table fit the
+// 160 KiB LDS of a CU, one 1024-thread workgroup per CU).
+#ifndef JC_TASK_SPAN
+#define JC_TASK_SPAN 896
+#endif
 constexpr int kSmallMax = 128;
-constexpr int kTaskSpan = 512;
+constexpr int kTaskSpan = JC_TASK_SPAN;
 
 struct EngineArgs {
   int32_t n;
@@ -62,12 +66,15 @@ hipError_t run_pairs(int mode, bool weighted, int32_t n, const int32_t* off, con
 hipError_t launch_fill_f32(float* x, int64_t n, float v, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* x, int64_t n, uint32_t v, hipStream_t s);
 hipError_t launch_fill_u64(uint64_t* x, int64_t n, uint64_t v, hipStream_t s);
+// Row sums take a scratch area of rowsum_scratch_bytes(n) (long-row list),
+// or nullptr to allocate one on the stream for the call.
+size_t rowsum_scratch_bytes(int32_t n);
 hipError_t launch_rowsum_f32(int32_t n, const int32_t* off, const int32_t* ind, const float* v,
-                             float* work, hipStream_t s);
+                             float* work, void* scratch, hipStream_t s);
 hipError_t launch_rowsum_f64(int32_t n, const int32_t* off, const int32_t* ind, const float* v,
-                             double* S, hipStream_t s);
+                             double* S, void* scratch, hipStream_t s);
 hipError_t launch_vertex_weights(int32_t n, const int32_t* off, const float* edge_w, float* W,
-                                 hipStream_t s);
+                                 void* scratch, hipStream_t s);
 hipError_t launch_jw_f32(int64_t m, const float* wi, const float* ws, float* out, hipStream_t s);
 hipError_t launch_jw_f64(int64_t m, const double* wi, const double* ws, float* out, hipStream_t s);
 hipError_t launch_ec_scan(int32_t n, const int32_t* off, int32_t* dest, float* wj, hipStream_t s);

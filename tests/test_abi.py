@@ -1,0 +1,81 @@
+"""C ABI of libjaccard_amd.so on the CPU: the library loads, exports every
+function include/jaccard_amd.h declares, and rejects bad arguments before
+touching a device (no compute calls here)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from jaccard_amd import _native, api
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "jaccard_amd.h")
+
+
+def declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(jc_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_reference_entry_points():
+    names = declared()
+    for must in ("jc_jaccard", "jc_jaccard_list", "jc_rowsum", "jc_fill", "jc_intersect", "jc_intersect_pairs",
+                 "jc_jw", "jc_ec_scan", "jc_ec_unweighted", "jc_vertex_weights", "jc_nonzero_count"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _native.load_jaccard_lib()
+    missing = [n for n in declared() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_header_compiles_as_c():
+    import subprocess
+    import tempfile
+    src = '#include "jaccard_amd.h"\nint main(void){ return JC_OK; }\n'
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "t.c")
+        open(p, "w").write(src)
+        subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.dirname(HEADER), "-c", p, "-o",
+                        os.path.join(d, "t.o")], check=True)
+
+
+def test_argument_validation_without_device():
+    lib = _native.load_jaccard_lib()
+    assert lib.jc_jaccard(None, None, None, 1, None) == api.ERR_INVALID_ARG
+    bad = api.JcCsr(-1, 0, None, None, None)
+    assert lib.jc_jaccard(ctypes.byref(bad), None, None, 1, None) == api.ERR_INVALID_ARG
+    nullind = api.JcCsr(4, 3, 1234, None, None)
+    assert lib.jc_jaccard(ctypes.byref(nullind), None, 5678, 1, None) == api.ERR_INVALID_ARG
+    empty = api.JcCsr(0, 0, None, None, None)
+    assert lib.jc_jaccard(ctypes.byref(empty), None, None, 1, None) == 0
+    assert lib.jc_jaccard_ex(ctypes.byref(empty), None, None, 1, 8, None, 0, None) == api.ERR_INVALID_ARG
+    assert lib.jc_jaccard_list(ctypes.byref(empty), None, -1, None, None, None, None) == api.ERR_INVALID_ARG
+    assert lib.jc_fill(-1, None, 0.0, None) == api.ERR_INVALID_ARG
+    assert lib.jc_jw(-2, None, None, None, None) == api.ERR_INVALID_ARG
+    assert lib.jc_rowsum(3, 0, None, None, None, None, None) == api.ERR_INVALID_ARG
+    assert lib.jc_nonzero_count(4, None, 1e-5, None, None) == api.ERR_INVALID_ARG
+
+
+def test_strerror_and_version():
+    lib = _native.load_jaccard_lib()
+    for code in (0, -1, -2, -3, -4, -5, -99):
+        assert lib.jc_strerror(code)
+    assert b"gfx950" in lib.jc_version()
+
+
+def test_workspace_size_grows_with_graph():
+    a = api.workspace_size(1000, 10000, False, True)
+    b = api.workspace_size(100000, 1000000, False, True)
+    c = api.workspace_size(100000, 1000000, True, False)
+    assert 0 < a < b < c
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    monkeypatch.setattr(_native, "_jaccard", None)
+    monkeypatch.setattr(_native, "JACCARD_LIB", str(tmp_path / "absent.so"))
+    with pytest.raises(_native.NativeLibraryMissing):
+        _native.load_jaccard_lib()

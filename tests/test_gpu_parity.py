@@ -249,3 +249,30 @@ def test_rmat16_c1(cuda):
     ref = oracle.ec_unweighted(g)
     assert_bits_equal(run(g, cuda, ec=True), ref, "rmat16 EC")
     assert_bits_equal(run(g, cuda, ec=False), ref, "rmat16 VC")
+
+
+GOLDEN = sorted(__import__("glob").glob(__import__("os").path.join(__import__("os").path.dirname(__file__),
+                                                                     "golden", "*.npz")))
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=[p.rsplit("/", 1)[-1][:-4] for p in GOLDEN])
+def test_golden_fixtures(path, cuda):
+    """The committed scipy/networkx fixtures, straight through the C ABI."""
+    import torch
+    z = np.load(path)
+    g = graphs.CSRGraph(n=int(z["n"]), offsets=z["offsets"], indices=z["indices"],
+                        weights=z["edge_w"] if "edge_w" in z.files else None)
+    print(f"[gpu] golden {path.rsplit('/', 1)[-1]} n={g.n} m={g.m}", flush=True)
+    for ec in (True, False):
+        assert_bits_equal(run(g, cuda, ec=ec), z["expected_unweighted"], f"golden unweighted ec={ec}")
+        got = run(g, cuda, z["vertex_w"], ec=ec)
+        assert rel_err(got, z["expected_weighted"]).max() <= WEIGHTED_RTOL
+    dg = api.DeviceCSR.from_host(g, cuda)
+    f = torch.from_numpy(z["first"]).to(cuda)
+    s = torch.from_numpy(z["second"]).to(cuda)
+    assert_bits_equal(api.jaccard_list(dg, f, s).cpu().numpy(), z["expected_pairs_unweighted"], "golden pairs")
+    pw = api.jaccard_list(dg, f, s, weights=torch.from_numpy(z["vertex_w"]).to(cuda)).cpu().numpy()
+    assert rel_err(pw, z["expected_pairs_weighted"]).max() <= WEIGHTED_RTOL
+    if g.weights is not None:
+        W = api.vertex_weights(dg).cpu().numpy()
+        np.testing.assert_allclose(W, z["vertex_w_expected"], rtol=6e-8)

@@ -1,0 +1,132 @@
+"""Pins the CPU oracle (oracle/jaccard_oracle.c) before it is trusted as the
+checker: committed golden fixtures (scipy/networkx, tests/golden), closed
+forms, brute force, and invariants. CPU only."""
+import glob
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import oracle
+from jaccard_amd import graphs
+
+import cases
+
+GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")))
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+def load(path):
+    z = np.load(path)
+    g = graphs.CSRGraph(n=int(z["n"]), offsets=z["offsets"], indices=z["indices"],
+                        weights=z["edge_w"] if "edge_w" in z.files else None)
+    return g, z
+
+
+def test_golden_files_present():
+    assert len(GOLDEN) >= 6
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p)[:-4] for p in GOLDEN])
+def test_oracle_matches_golden(path):
+    g, z = load(path)
+    want = z["expected_unweighted"]
+    assert np.array_equal(bits(oracle.ec_unweighted(g)), bits(want))
+    assert np.array_equal(bits(oracle.vc_unweighted(g)), bits(want))
+    W = z["vertex_w"].astype(np.float64)
+    np.testing.assert_allclose(oracle.weighted_f64(g, W), z["expected_weighted"], rtol=1e-12, atol=0)
+    pu = oracle.pairs_unweighted(g, z["first"], z["second"])
+    assert np.array_equal(bits(pu), bits(z["expected_pairs_unweighted"]))
+    pw = oracle.pairs_weighted_f64(g, W, z["first"], z["second"])
+    np.testing.assert_allclose(pw, z["expected_pairs_weighted"], rtol=1e-12, atol=0, equal_nan=True)
+    if g.weights is not None:
+        np.testing.assert_allclose(oracle.vertex_weights(g), z["vertex_w_expected"], rtol=1e-12)
+
+
+@pytest.mark.parametrize("n", [3, 5, 17, 130])
+def test_complete_graph_closed_form(n):
+    g = cases.complete(n)
+    want = np.float32(n - 2) / np.float32(2 * (n - 1) - (n - 2))  # I = n-2, d = n-1
+    out = oracle.ec_unweighted(g)
+    assert np.all(bits(out) == bits(np.full(g.m, want, np.float32)))
+    assert abs(float(want) - (n - 2) / n) < 1e-6
+    # uniform vertex weight w: W-weighted Jaccard equals the unweighted one
+    J = oracle.weighted_f64(g, np.full(n, 0.375))
+    np.testing.assert_allclose(J, (n - 2) / n, rtol=1e-14)
+
+
+@pytest.mark.parametrize("g", [cases.star(50), cases.path(40), cases.cycle(33), cases.complete_bipartite(4, 7)],
+                         ids=["star", "path", "cycle", "K4x7"])
+def test_triangle_free_graphs_are_zero(g):
+    assert not oracle.ec_unweighted(g).any()
+    assert not oracle.vc_unweighted(g).any()
+
+
+def test_two_triangles_hand_computed():
+    g = cases.two_triangles()
+    rows = np.repeat(np.arange(g.n), np.diff(g.offsets))
+    want = {(0, 1): 0.25, (0, 2): 0.25, (1, 2): 0.5, (1, 3): 0.25, (2, 3): 0.25}
+    out = oracle.ec_unweighted(g)
+    for k, (u, v) in enumerate(zip(rows, g.indices)):
+        assert out[k] == want[(min(u, v), max(u, v))]
+
+
+def brute(g):
+    A = sp.csr_matrix((np.ones(g.m), g.indices, g.offsets), shape=(g.n, g.n))
+    I = np.asarray((A @ A.T)[np.repeat(np.arange(g.n), np.diff(g.offsets)), g.indices]).ravel()
+    return I
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_brute_force_rmat(seed):
+    g = graphs.rmat(10, 12, seed=seed)
+    I = brute(g).astype(np.float32)
+    deg = np.diff(g.offsets)
+    rows = np.repeat(np.arange(g.n), deg)
+    want = I / ((deg[rows] + deg[g.indices]).astype(np.float32) - I)
+    assert np.array_equal(bits(oracle.ec_unweighted(g)), bits(want))
+
+
+def test_ec_edges_subset_matches_full():
+    g = graphs.rmat(10, 16, seed=3)
+    full = oracle.ec_unweighted(g)
+    rng = np.random.default_rng(0)
+    e = rng.integers(0, g.m, 500)
+    assert np.array_equal(bits(oracle.ec_edges(g, e)), bits(full[e]))
+
+
+def test_symmetry_and_range():
+    g = graphs.chung_lu(n=4096, mean_deg=16, max_deg=1024, seed=5)
+    out = oracle.ec_unweighted(g)
+    rows = np.repeat(np.arange(g.n), np.diff(g.offsets))
+    # J(u,v) == J(v,u) bitwise across the two stored directions
+    key = rows.astype(np.int64) * g.n + g.indices
+    rkey = g.indices.astype(np.int64) * g.n + rows
+    order = np.argsort(key)
+    pos = order[np.searchsorted(key[order], rkey)]
+    assert np.array_equal(bits(out), bits(out[pos]))
+    assert out.min() >= 0 and out.max() <= 1
+
+
+def test_weighted_scale_invariance():
+    g = graphs.rmat(9, 16, seed=4)
+    rng = np.random.default_rng(1)
+    W = rng.random(g.n) + 0.5
+    a = oracle.weighted_f64(g, W)
+    b = oracle.weighted_f64(g, 8.0 * W)  # exact scaling by a power of two
+    np.testing.assert_allclose(a, b, rtol=1e-13)
+
+
+def test_nonzero_count_threshold():
+    out = np.array([0.0, 1e-6, 1e-5, 2e-5, 0.5], np.float32)
+    # reference counts entries strictly greater than 1e-5f (jaccard.cpp:811-819)
+    assert oracle.nonzero_count(out) == 2
+
+
+def test_oracle_thread_count_independent():
+    g = graphs.rmat(10, 16, seed=9)
+    assert np.array_equal(bits(oracle.ec_unweighted(g, threads=1)), bits(oracle.ec_unweighted(g, threads=4)))

@@ -95,6 +95,22 @@ int jc_jaccard_ex(const jc_csr *g, const float *vertex_weights, float *result, i
                   unsigned flags, void *workspace, size_t workspace_bytes, jc_stream_t stream);
 size_t jc_jaccard_workspace_size(int32_t n, int32_t m, int weighted, int edge_centric);
 
+/* Launch tuning for jc_jaccard_tuned. None of these change results; zero
+ * fields take the defaults. No reference counterpart (the reference's launch
+ * shapes are fixed, jaccard.cpp:264-276, 403-418, 745-753). */
+typedef struct jc_tuning {
+  int32_t small_max;           /* degree bin threshold: rows up to this degree run one
+                                  wave per row, longer rows one workgroup per task
+                                  (1..128, default 128) */
+  int32_t small_blocks_per_cu; /* grid of the one-wave-per-row kernel (default 16) */
+  int32_t reserved[6];
+} jc_tuning;
+
+/* jc_jaccard_ex with launch tuning (NULL = defaults). */
+int jc_jaccard_tuned(const jc_csr *g, const float *vertex_weights, float *result, int edge_centric,
+                     unsigned flags, void *workspace, size_t workspace_bytes, const jc_tuning *tuning,
+                     jc_stream_t stream);
+
 /* sygraph::jaccard_list<int, int, float>(graph, [weights,] num_pairs, first, second, result, q)
  *   declarations standalone_algorithms.hpp:109-112, :131-133; definitions
  *   jaccard.cpp:957-988 via jaccard_pairs jaccard.cpp:901-923.
@@ -111,6 +127,13 @@ int jc_jaccard_list(const jc_csr *g, const float *vertex_weights, int32_t num_pa
 /* Weighted-VC preamble of the reference driver, main.cpp:211-275:
  * W[k] = sum of the edge weights of row k (g->edge_w, required). */
 int jc_vertex_weights(const jc_csr *g, float *W, jc_stream_t stream);
+
+/* jc_vertex_weights with a caller workspace of at least
+ * jc_vertex_weights_workspace_size(n) bytes: allocates nothing (for stream
+ * capture). No reference counterpart. */
+int jc_vertex_weights_ex(const jc_csr *g, float *W, void *workspace, size_t workspace_bytes,
+                         jc_stream_t stream);
+size_t jc_vertex_weights_workspace_size(int32_t n);
 
 /* "non zero pairs" count of the EC path, jaccard.cpp:811-819: the number of
  * result entries greater than thresh (the reference uses 1e-5f).

@@ -80,7 +80,7 @@ struct AsyncBuf {
 };
 
 int run_jaccard(const jc_csr* g, const float* vw, float* result, bool ec, unsigned flags,
-                unsigned char* ws, hipStream_t s) {
+                unsigned char* ws, const jc_tuning* tuning, hipStream_t s) {
   const bool weighted = vw != nullptr;
   const bool fixup = (flags & JC_FLAG_ASSUME_SYMMETRIC) == 0;
   const Layout L = layout(g->n, g->m, weighted, ec);
@@ -90,6 +90,8 @@ int run_jaccard(const jc_csr* g, const float* vw, float* result, bool ec, unsign
   a.off = g->offsets;
   a.ind = g->indices;
   a.vw = vw;
+  a.small_max = tuning ? tuning->small_max : 0;
+  a.small_blocks_per_cu = tuning ? tuning->small_blocks_per_cu : 0;
   double* S64 = weighted ? reinterpret_cast<double*>(ws + L.s64) : nullptr;
   if (weighted) JC_TRY(jc::launch_rowsum_f64(g->n, g->offsets, g->indices, vw, S64, ws + L.rows, s));
   a.S64 = S64;
@@ -132,8 +134,9 @@ size_t jc_jaccard_workspace_size(int32_t n, int32_t m, int weighted, int edge_ce
   return layout(n, m, weighted != 0, edge_centric != 0).total;
 }
 
-int jc_jaccard_ex(const jc_csr* g, const float* vertex_weights, float* result, int edge_centric,
-                  unsigned flags, void* workspace, size_t workspace_bytes, jc_stream_t stream) {
+int jc_jaccard_tuned(const jc_csr* g, const float* vertex_weights, float* result, int edge_centric,
+                     unsigned flags, void* workspace, size_t workspace_bytes, const jc_tuning* tuning,
+                     jc_stream_t stream) {
   if (!csr_args_ok(g) || (g->m > 0 && !result) || (flags & ~JC_FLAG_ASSUME_SYMMETRIC)) return JC_ERR_INVALID_ARG;
   if (g->n == 0 || g->m == 0) return JC_OK;
   const bool ec = edge_centric != 0;
@@ -141,12 +144,18 @@ int jc_jaccard_ex(const jc_csr* g, const float* vertex_weights, float* result, i
   hipStream_t s = hs(stream);
   if (workspace) {
     if (workspace_bytes < need) return JC_ERR_WORKSPACE;
-    return run_jaccard(g, vertex_weights, result, ec, flags, static_cast<unsigned char*>(workspace), s);
+    return run_jaccard(g, vertex_weights, result, ec, flags, static_cast<unsigned char*>(workspace), tuning, s);
   }
   AsyncBuf buf;
   buf.s = s;
   JC_TRY(hipMallocAsync(&buf.p, need, s));
-  return run_jaccard(g, vertex_weights, result, ec, flags, static_cast<unsigned char*>(buf.p), s);
+  return run_jaccard(g, vertex_weights, result, ec, flags, static_cast<unsigned char*>(buf.p), tuning, s);
+}
+
+int jc_jaccard_ex(const jc_csr* g, const float* vertex_weights, float* result, int edge_centric,
+                  unsigned flags, void* workspace, size_t workspace_bytes, jc_stream_t stream) {
+  return jc_jaccard_tuned(g, vertex_weights, result, edge_centric, flags, workspace, workspace_bytes, nullptr,
+                          stream);
 }
 
 int jc_jaccard(const jc_csr* g, const float* vertex_weights, float* result, int edge_centric,
@@ -173,6 +182,20 @@ int jc_vertex_weights(const jc_csr* g, float* W, jc_stream_t stream) {
     return JC_OK;
   }
   JC_TRY(jc::launch_vertex_weights(g->n, g->offsets, g->edge_w, W, nullptr, hs(stream)));
+  return JC_OK;
+}
+
+size_t jc_vertex_weights_workspace_size(int32_t n) { return n < 0 ? 0 : al(jc::rowsum_scratch_bytes(n)); }
+
+int jc_vertex_weights_ex(const jc_csr* g, float* W, void* workspace, size_t workspace_bytes, jc_stream_t stream) {
+  if (!csr_args_ok(g) || (g->n > 0 && !W) || (g->m > 0 && !g->edge_w) || !workspace) return JC_ERR_INVALID_ARG;
+  if (g->n == 0) return JC_OK;
+  if (workspace_bytes < jc_vertex_weights_workspace_size(g->n)) return JC_ERR_WORKSPACE;
+  if (g->m == 0) {
+    JC_TRY(jc::launch_fill_f32(W, g->n, 0.0f, hs(stream)));
+    return JC_OK;
+  }
+  JC_TRY(jc::launch_vertex_weights(g->n, g->offsets, g->edge_w, W, workspace, hs(stream)));
   return JC_OK;
 }
 
@@ -226,6 +249,8 @@ int jc_intersect(int32_t n, int32_t m, const int32_t* offsets, const int32_t* in
   a.out32 = weight_i;
   a.ws32 = weight_s;
   a.accumulate = 1;
+  a.small_max = 0;
+  a.small_blocks_per_cu = 0;
   JC_TRY(jc::launch_fill_u32(reinterpret_cast<uint32_t*>(weight_s), m, jc::kSentinel32, s));
   JC_TRY(jc::run_engine(jc::kModeIsF32, v != nullptr, a, buf.p, true, s));
   return JC_OK;

@@ -827,14 +827,14 @@ __device__ __forceinline__ int bucket_of(int d) { return 31 - __clz(d); }
 
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_sched_count(int32_t n, const int32_t* __restrict__ off,
-                                                       Sched* sc) {
+                                                       Sched* sc, int small_max) {
   __shared__ int hist[kBuckets + 1];
   for (int i = threadIdx.x; i < kBuckets + 1; i += BLOCK) hist[i] = 0;
   __syncthreads();
   for (int32_t u = blockIdx.x * BLOCK + threadIdx.x; u < n; u += gridDim.x * BLOCK) {
     const int d = off[u + 1] - off[u];
     if (d <= 0) continue;
-    if (d <= kSmallMax)
+    if (d <= small_max)
       atomicAdd(&hist[kBuckets], 1);
     else
       atomicAdd(&hist[bucket_of(d)], (d + kTaskSpan - 1) / kTaskSpan);
@@ -864,7 +864,7 @@ __global__ void k_sched_scan(Sched* sc) {
 
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_sched_emit(int32_t n, const int32_t* __restrict__ off,
-                                                      Sched* sc, int2* tasks, int32_t* small_rows) {
+                                                      Sched* sc, int2* tasks, int32_t* small_rows, int small_max) {
   __shared__ int hist[kBuckets + 1];
   __shared__ int basep[kBuckets + 1];
   for (int32_t u0 = blockIdx.x * BLOCK; u0 < n; u0 += gridDim.x * BLOCK) {
@@ -875,7 +875,7 @@ __global__ __launch_bounds__(BLOCK) void k_sched_emit(int32_t n, const int32_t* 
     if (u < n) {
       d = off[u + 1] - off[u];
       if (d > 0) {
-        if (d <= kSmallMax) {
+        if (d <= small_max) {
           b = kBuckets;
           cntv = 1;
         } else {
@@ -905,8 +905,9 @@ __global__ __launch_bounds__(BLOCK) void k_sched_emit(int32_t n, const int32_t* 
 static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 static int64_t max_tasks(int32_t n, int64_t m) {
-  // each row with degree > kSmallMax contributes ceil(d / kTaskSpan) <= d / kTaskSpan + 1 tasks
-  return m / kTaskSpan + std::min<int64_t>(n, m / (kSmallMax + 1)) + 1;
+  // each row with degree > small_max (>= 1) contributes ceil(d / kTaskSpan) <= d / kTaskSpan + 1
+  // tasks, and there are at most min(n, m / 2) such rows
+  return m / kTaskSpan + std::min<int64_t>(n, m / 2) + 1;
 }
 
 size_t engine_workspace_bytes(int32_t n, int64_t m) {
@@ -922,6 +923,8 @@ static hipError_t run_engine_t(const EngineArgs& base, void* ws, bool fixup, hip
   constexpr int BB = JC_BLOCK;  // block kernel block
   constexpr int PB = 256;   // schedule kernels
   EngineArgs a = base;
+  if (a.small_max < 1 || a.small_max > kSmallMax) a.small_max = kSmallMax;
+  if (a.small_blocks_per_cu < 1) a.small_blocks_per_cu = 16;
   unsigned char* p = static_cast<unsigned char*>(ws);
   Sched* sc = reinterpret_cast<Sched*>(p);
   p += align_up(sizeof(Sched), 256);
@@ -942,9 +945,9 @@ static hipError_t run_engine_t(const EngineArgs& base, void* ws, bool fixup, hip
     const int g = std::min<int64_t>((a.n + PB - 1) / PB, (int64_t)cus * 8);
     {
       ProfScope ps("sched", stream);
-      hipLaunchKernelGGL(k_sched_count<PB>, dim3(g), dim3(PB), 0, stream, a.n, a.off, sc);
+      hipLaunchKernelGGL(k_sched_count<PB>, dim3(g), dim3(PB), 0, stream, a.n, a.off, sc, a.small_max);
       hipLaunchKernelGGL(k_sched_scan, dim3(1), dim3(64), 0, stream, sc);
-      hipLaunchKernelGGL(k_sched_emit<PB>, dim3(g), dim3(PB), 0, stream, a.n, a.off, sc, tasks, small);
+      hipLaunchKernelGGL(k_sched_emit<PB>, dim3(g), dim3(PB), 0, stream, a.n, a.off, sc, tasks, small, a.small_max);
       if ((err = hipGetLastError()) != hipSuccess) return err;
     }
 
@@ -961,7 +964,7 @@ static hipError_t run_engine_t(const EngineArgs& base, void* ws, bool fixup, hip
       hipLaunchKernelGGL((k_owner_block<MODE, W, BB>), dim3(cus * occ), dim3(BB), Cfg::lds_bytes, stream, a);
       if ((err = hipGetLastError()) != hipSuccess) return err;
     }
-    const int sg = std::min<int64_t>((a.n + (SB / kWave) - 1) / (SB / kWave), (int64_t)cus * 16);
+    const int sg = std::min<int64_t>((a.n + (SB / kWave) - 1) / (SB / kWave), (int64_t)cus * a.small_blocks_per_cu);
     {
       ProfScope ps("owner_small", stream);
       hipLaunchKernelGGL((k_owner_small<MODE, W, SB>), dim3(sg), dim3(SB), 0, stream, a);

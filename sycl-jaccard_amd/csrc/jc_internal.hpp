@@ -41,8 +41,11 @@ struct EngineArgs {
   const int2* tasks;     // (row, piece) for rows with degree > kSmallMax, degree-descending buckets
   const int* num_tasks;
   int* task_counter;
-  const int32_t* small_rows;  // rows with 1 <= degree <= kSmallMax
+  const int32_t* small_rows;  // rows with 1 <= degree <= small_max
   const int* num_small;
+  // tuning (results do not depend on these)
+  int small_max;              // degree bin threshold: one wave per row up to here (1..kSmallMax)
+  int small_blocks_per_cu;    // grid of the one-wave-per-row kernel
 };
 
 // Workspace needed by an owner-engine pass over a graph with n vertices and m

@@ -251,6 +251,62 @@ const int32_t* jcg_indices(const jcg_graph* g) { return g->indices.data(); }
 const float* jcg_weights(const jcg_graph* g) { return g->weights.empty() ? nullptr : g->weights.data(); }
 void jcg_free(jcg_graph* g) { delete g; }
 
+// A batch of RMAT graphs packed as one block-diagonal CSR (the disjoint
+// union, global vertex ids): graph i has n[i] vertices, round(ef[i] * n[i])
+// draws from seed seeds[i], edge weights when weighted != 0 (weight seed
+// seeds[i] ^ wxor). vbase[i] / ebase[i] (count + 1 entries, caller-allocated)
+// receive each graph's first vertex / first stored edge in the union.
+// Graphs are generated in parallel, largest first. Returns nullptr if the
+// union does not fit int32 (> 2^31 - 1 vertices or stored edges).
+jcg_graph* jcg_rmat_batch(int32_t count, const int32_t* n, const double* ef, const uint64_t* seeds, int weighted,
+                          uint64_t wxor, double a, double b, double c, int64_t* vbase, int64_t* ebase) {
+  if (count <= 0) return nullptr;
+  std::vector<jcg_graph*> parts((size_t)count, nullptr);
+  std::vector<int32_t> order((size_t)count);
+  for (int32_t i = 0; i < count; ++i) order[i] = i;
+  std::sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return (double)n[x] * ef[x] > (double)n[y] * ef[y]; });
+  const int nested = omp_get_max_active_levels();
+  omp_set_max_active_levels(1);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int32_t t = 0; t < count; ++t) {
+    const int32_t i = order[t];
+    int scale = 1;
+    while ((1ll << scale) < n[i]) ++scale;
+    parts[i] = jcg_rmat(scale, n[i], ef[i], a, b, c, seeds[i], weighted, seeds[i] ^ wxor);
+  }
+  omp_set_max_active_levels(nested);
+  bool ok = true;
+  vbase[0] = 0;
+  ebase[0] = 0;
+  for (int32_t i = 0; i < count; ++i) {
+    if (!parts[i]) ok = false;
+    vbase[i + 1] = vbase[i] + (parts[i] ? parts[i]->n : 0);
+    ebase[i + 1] = ebase[i] + (parts[i] ? parts[i]->m : 0);
+  }
+  if (!ok || vbase[count] > INT32_MAX || ebase[count] > INT32_MAX) {
+    for (auto* p : parts) delete p;
+    return nullptr;
+  }
+  auto* u = new jcg_graph;
+  u->n = (int32_t)vbase[count];
+  u->m = ebase[count];
+  u->weighted = weighted != 0;
+  u->offsets.resize((size_t)u->n + 1);
+  u->indices.resize((size_t)u->m);
+  if (u->weighted) u->weights.resize((size_t)u->m);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int32_t i = 0; i < count; ++i) {
+    const jcg_graph* g = parts[i];
+    const int32_t vb = (int32_t)vbase[i], eb = (int32_t)ebase[i];
+    for (int32_t r = 0; r < g->n; ++r) u->offsets[vb + r] = eb + g->offsets[r];
+    for (int64_t k = 0; k < g->m; ++k) u->indices[eb + k] = vb + g->indices[k];
+    if (u->weighted) std::memcpy(u->weights.data() + eb, g->weights.data(), sizeof(float) * g->m);
+  }
+  u->offsets[u->n] = (int32_t)u->m;
+  for (auto* p : parts) delete p;
+  return u;
+}
+
 // The per-pair weight function, exported so tests can check symmetry/range.
 float jcg_pair_weight(uint64_t wseed, int32_t u, int32_t v) { return pair_weight(wseed, u, v); }
 

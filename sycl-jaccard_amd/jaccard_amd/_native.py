@@ -63,6 +63,8 @@ def load_graph_lib() -> ctypes.CDLL:
             lib.jcg_free.argtypes = [vp]
             lib.jcg_pair_weight.restype = ctypes.c_float
             lib.jcg_pair_weight.argtypes = [u64, i32, i32]
+            lib.jcg_rmat_batch.restype = vp
+            lib.jcg_rmat_batch.argtypes = [i32, vp, vp, vp, ctypes.c_int, u64, f64, f64, f64, vp, vp]
             lib.jcg_stats.restype = None
             lib.jcg_stats.argtypes = [i32, vp, vp, ctypes.POINTER(i64)]
             _graph = lib
@@ -77,8 +79,11 @@ def _declare_jaccard(lib: ctypes.CDLL) -> None:
         "jc_jaccard": [vp, vp, vp, I, vp],
         "jc_jaccard_ex": [vp, vp, vp, I, u32, vp, sz, vp],
         "jc_jaccard_workspace_size": [i32, i32, I, I],
+        "jc_jaccard_tuned": [vp, vp, vp, I, u32, vp, sz, vp, vp],
         "jc_jaccard_list": [vp, vp, i32, vp, vp, vp, vp],
         "jc_vertex_weights": [vp, vp, vp],
+        "jc_vertex_weights_ex": [vp, vp, vp, sz, vp],
+        "jc_vertex_weights_workspace_size": [i32],
         "jc_nonzero_count": [i32, vp, f32, ctypes.POINTER(i64), vp],
         "jc_fill": [i64, vp, f32, vp],
         "jc_rowsum": [i32, i32, vp, vp, vp, vp, vp],
@@ -102,6 +107,7 @@ def _declare_jaccard(lib: ctypes.CDLL) -> None:
         f.argtypes = args
         f.restype = I
     lib.jc_jaccard_workspace_size.restype = sz
+    lib.jc_vertex_weights_workspace_size.restype = sz
     lib.jc_strerror.restype = ctypes.c_char_p
     lib.jc_version.restype = ctypes.c_char_p
 

@@ -157,6 +157,20 @@ def vertex_weights(graph: DeviceCSR, out=None, stream=None):
     return out
 
 
+def vertex_weights_ex(graph: DeviceCSR, out, workspace, stream=None):
+    """vertex_weights with a caller workspace (allocates nothing)."""
+    lib = load_jaccard_lib()
+    g = graph.c_struct()
+    _check(lib.jc_vertex_weights_ex(ctypes.byref(g), _ptr(out), workspace.data_ptr(),
+                                    workspace.numel() * workspace.element_size(), _stream(stream)),
+           "jc_vertex_weights_ex")
+    return out
+
+
+def vertex_weights_workspace_size(n: int) -> int:
+    return int(load_jaccard_lib().jc_vertex_weights_workspace_size(n))
+
+
 def nonzero_count(result, thresh: float = 1e-5, stream=None) -> int:
     """Entries above ``thresh`` (the EC path's "non zero pairs" line)."""
     lib = load_jaccard_lib()

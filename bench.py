@@ -38,6 +38,22 @@ WORKLOADS = {
 }
 
 
+# rocprofv3 FETCH_SIZE/WRITE_SIZE summary of the dominant kernel per workload,
+# scaled to bytes by a copy kernel of known size (see the file for the commands).
+TRAFFIC_PROFILES = {"rmat22_weighted": "profiles/owner_block_rmat22_weighted_pmc.md"}
+
+
+def measured_traffic(workload):
+    """HBM bytes per k_owner_block launch from the committed counter summary, or None."""
+    rel = TRAFFIC_PROFILES.get(workload)
+    if rel is None or not os.path.exists(os.path.join(ROOT, rel)):
+        return None
+    for line in open(os.path.join(ROOT, rel)):
+        if line.startswith("traffic_bytes_per_launch"):
+            return int(line.split("=")[1])
+    return None
+
+
 def make_graph(spec, weighted):
     from jaccard_amd import graphs
     kind = spec["kind"]
@@ -137,6 +153,9 @@ def main():
     ap.add_argument("--batched", action="store_true", help="also run the batched workload (4096 graphs)")
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--batch-reps", type=int, default=3)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="also run 3 device-to-device copies of BYTES (a known-size kernel that "
+                         "calibrates FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
@@ -209,6 +228,15 @@ def main():
     if args.batched:
         extras["batched"] = run_batched(args)
 
+    if args.calibrate_copy:
+        src = torch.ones(args.calibrate_copy // 4, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        extras["calibration_copy_bytes"] = args.calibrate_copy
+        del src, dst
+
     cpu = None
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline...", file=sys.stderr, flush=True)
@@ -221,7 +249,7 @@ def main():
         "config": {"workload": args.workload, "description": desc, "algorithm": "edge_centric" if ec else "vertex_centric",
                    "n": g.n, "m": g.m, "weighted": weighted, **{k: v for k, v in spec.items() if k != "kind"}},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None, "kernel": "k_owner_block",
+                     "frac": achieved / HBM_PEAK, "traffic": measured_traffic(args.workload), "kernel": "k_owner_block",
                      "algorithmic_bytes": B},
         "cpu_baseline": cpu,
         "extras": extras,

@@ -140,6 +140,58 @@ def run_batched(args):
             "coeffs_per_s_batched": (bu.union.m + bw.union.m) / batched_s}
 
 
+def run_served(args):
+    """Serving workload: 64 resident RMAT graphs (scales 14..20 cycling, edge
+    factor 16, float32 weights in (0,1]); requests pick a graph uniformly and are
+    whole-graph weighted Jaccard (probability 1/2) or k vertex pairs (k
+    log-uniform in [1, 65536], half stored edges, half uniform pairs).
+    Continuous batching vs one plain launch per request (FIFO), first as a burst
+    (all requests at t=0: throughput), then as a Poisson stream at a fixed QPS
+    below the plain path's capacity (latency)."""
+    import numpy as np
+    import torch
+    from jaccard_amd import serve
+    t0 = time.time()
+    gs = serve.serving_graphs(count=64, seed=7)
+    dev = torch.device("cuda:0")
+    dg = serve.upload(gs, dev, weighted=True)
+    print(f"[bench] serving: 64 graphs, m={sum(g.m for g in gs)}, setup {time.time() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    cont = serve.Server(gs, dev, mode=serve.CONTINUOUS, slots=4, hbm_budget_bytes=1 << 30, device_graphs=dg)
+    fifo = serve.Server(gs, dev, mode=serve.FIFO, device_graphs=dg)
+    out = {"graphs": 64, "stored_edges": int(sum(g.m for g in gs)), "arena_bytes": cont.arena_bytes(),
+           "hbm_budget_bytes": 1 << 30, "slots": 4}
+
+    def both(reqs, tag):
+        res_c = serve.result_buffers(gs, reqs, dev, share_full=True)
+        res_f = serve.result_buffers(gs, reqs, dev, share_full=True)
+        cont.run(reqs[: min(8, len(reqs))], res_c[: min(8, len(reqs))])  # warm
+        st_c, info_c = cont.run(reqs, res_c)
+        st_f, info_f = fifo.run(reqs, res_f)
+        assert (st_c["status"] == 0).all() and (st_f["status"] == 0).all()
+        torch.cuda.synchronize()
+        same = all(np.array_equal(a.cpu().numpy().view(np.uint32), b.cpu().numpy().view(np.uint32))
+                   for a, b in zip(res_c, res_f))
+        c, f = serve.latency_summary(reqs, st_c), serve.latency_summary(reqs, st_f)
+        c.update(batches=info_c["batches"], max_in_flight=info_c["max_in_flight"],
+                 peak_budget_bytes=info_c["peak_budget_bytes"])
+        print(f"[bench] serving {tag}: continuous {c['requests_per_s']:.1f} req/s p50 {c['p50_ms']:.2f} "
+              f"p99 {c['p99_ms']:.2f} ms | plain {f['requests_per_s']:.1f} req/s p50 {f['p50_ms']:.2f} "
+              f"p99 {f['p99_ms']:.2f} ms | identical {same}", file=sys.stderr, flush=True)
+        return {"continuous": c, "plain": f, "identical_outputs": bool(same),
+                "throughput_ratio": c["requests_per_s"] / f["requests_per_s"]}
+
+    burst = serve.request_stream(gs, args.serve_requests, qps=0.0, seed=11)
+    out["burst"] = both(burst, "burst")
+    qps = args.serve_qps or 0.7 * out["burst"]["plain"]["requests_per_s"]
+    stream = serve.request_stream(gs, args.serve_requests, qps=qps, seed=12)
+    out["poisson"] = both(stream, f"poisson {qps:.0f} qps")
+    out["poisson"]["qps"] = qps
+    cont.close()
+    fifo.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,6 +205,9 @@ def main():
     ap.add_argument("--batched", action="store_true", help="also run the batched workload (4096 graphs)")
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--batch-reps", type=int, default=3)
+    ap.add_argument("--served", action="store_true", help="also run the serving workload")
+    ap.add_argument("--serve-requests", type=int, default=512)
+    ap.add_argument("--serve-qps", type=float, default=0.0, help="Poisson rate (0: 0.7 x plain capacity)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="also run 3 device-to-device copies of BYTES (a known-size kernel that "
                          "calibrates FETCH_SIZE/WRITE_SIZE counters)")
@@ -227,6 +282,9 @@ def main():
 
     if args.batched:
         extras["batched"] = run_batched(args)
+
+    if args.served:
+        extras["served"] = run_served(args)
 
     if args.calibrate_copy:
         src = torch.ones(args.calibrate_copy // 4, dtype=torch.float32, device=dev)

@@ -206,6 +206,82 @@ int jc_profile_reset(void);
 int jc_profile_query(const char *kernel, int *count, double *total_ms);
 int jc_profile_names(char *buf, size_t len);
 
+/* ---- Serving (no reference counterpart: the reference computes one graph
+ * per call, jaccard.cpp:926-955) ----------------------------------------
+ *
+ * A server keeps a set of graphs resident in HBM as one arena (their disjoint
+ * union with global ids) and replays a stream of timed requests against it:
+ * whole-graph edge-centric Jaccard, or the Jaccard of k vertex pairs. In
+ * JC_SERVE_CONTINUOUS mode requests that arrive while others run are admitted
+ * into the next batch as soon as a slot (a HIP stream) and memory from the
+ * HBM budget are free: one batch runs the whole-graph requests of several
+ * graphs in one owner-engine launch over their row ranges and all pair
+ * requests in one pair launch, and up to `slots` batches overlap on the
+ * device. JC_SERVE_FIFO serves one request at a time in arrival order with
+ * the plain entry points (jc_jaccard, jc_jaccard_list) on each graph's own
+ * CSR, the per-request baseline. Both modes give the same bits for the same
+ * request (exact accumulation; per-row reductions independent of batching). */
+enum { JC_SERVE_CONTINUOUS = 0, JC_SERVE_FIFO = 1 };
+enum { JC_REQ_FULL = 0, JC_REQ_PAIRS = 1 };
+
+typedef struct jc_server_config {
+  int32_t slots;              /* batches in flight at once, one stream each (1..16) */
+  int32_t max_batch_requests; /* requests per batch (>= 1) */
+  int64_t max_batch_edges;    /* stored edges of whole-graph work per batch; a
+                                 single larger graph still runs alone */
+  int64_t max_batch_pairs;    /* pairs per batch; a single larger request runs alone */
+  size_t hbm_budget_bytes;    /* device memory for batch workspaces and pair staging;
+                                 a request is admitted only when its batch fits */
+  int32_t mode;               /* JC_SERVE_CONTINUOUS or JC_SERVE_FIFO */
+  int32_t reserved[5];
+} jc_server_config;
+
+typedef struct jc_server jc_server;
+
+/* Builds the resident arena from num_graphs symmetric CSR graphs (device
+ * pointers, local ids; edge_w unused) and their vertex weights (device
+ * arrays of n floats, or vertex_weights = NULL for unweighted Jaccard), and
+ * allocates the HBM budget. Synchronous. The graphs must stay allocated for
+ * the server's lifetime (JC_SERVE_FIFO reads them). */
+int jc_server_create(int32_t num_graphs, const jc_csr *graphs, const float *const *vertex_weights,
+                     const jc_server_config *cfg, jc_server **out);
+int jc_server_destroy(jc_server *s);
+/* Bytes of device memory the arena holds (outside the budget). */
+size_t jc_server_arena_bytes(const jc_server *s);
+
+typedef struct jc_request {
+  int32_t graph;         /* index into the graphs given to jc_server_create */
+  int32_t kind;          /* JC_REQ_FULL or JC_REQ_PAIRS */
+  int32_t num_pairs;     /* JC_REQ_PAIRS: k */
+  int32_t reserved;
+  const int32_t *first;  /* JC_REQ_PAIRS: HOST arrays of k local vertex ids (ids */
+  const int32_t *second; /*   outside the graph give NaN) */
+  float *result;         /* DEVICE output: m floats (FULL) or k floats (PAIRS) */
+  double arrival;        /* seconds after the start of jc_server_run */
+} jc_request;
+
+typedef struct jc_request_stats {
+  double admitted;       /* seconds after start: work enqueued */
+  double completed;      /* seconds after start: result in `result` */
+  int32_t batch;         /* batch number (JC_SERVE_FIFO: request order) */
+  int32_t status;        /* JC_OK or a JC_ERR_* code for this request */
+} jc_request_stats;
+
+typedef struct jc_server_run_info {
+  int32_t batches;           /* batches launched */
+  int32_t max_in_flight;     /* most batches in flight at once */
+  size_t peak_budget_bytes;  /* most budget memory held at once */
+  double wall_seconds;       /* start to last completion */
+} jc_server_run_info;
+
+/* Replays `count` requests: each becomes visible at its arrival time (host
+ * monotonic clock from the start of the call) and is served in the server's
+ * mode. Returns when all are complete; stats[i] describes reqs[i]. A request
+ * that can never fit the budget gets status JC_ERR_OUT_OF_MEMORY. Synchronous;
+ * one call at a time per server. */
+int jc_server_run(jc_server *s, const jc_request *reqs, int32_t count, jc_request_stats *stats,
+                  jc_server_run_info *info);
+
 const char *jc_strerror(int code);
 int jc_last_hip_error(void);
 const char *jc_version(void);

@@ -193,11 +193,14 @@ void or_weighted_f64(int32_t n, int64_t m, const int32_t *off, const int32_t *in
  * intersection size held in fp32, the per-pair form of the EC arithmetic. */
 void or_pairs_unweighted(int32_t n, const int32_t *off, const int32_t *ind, int64_t np,
                          const int32_t *first, const int32_t *second, float *out, int threads) {
-  (void)n;
   set_threads(threads);
 #pragma omp parallel for schedule(dynamic, 256)
   for (int64_t i = 0; i < np; i++) {
     int32_t a = first[i], b = second[i];
+    if (a < 0 || a >= n || b < 0 || b >= n) { /* ids outside the graph: NaN */
+      out[i] = NAN;
+      continue;
+    }
     int32_t da = off[a + 1] - off[a], db = off[b + 1] - off[b];
     float I = (float)merge_count(ind + off[a], da, ind + off[b], db);
     out[i] = I / ((float)(da + db) - I);
@@ -208,11 +211,14 @@ void or_pairs_unweighted(int32_t n, const int32_t *off, const int32_t *ind, int6
 void or_pairs_weighted_f64(int32_t n, const int32_t *off, const int32_t *ind, const double *v,
                            int64_t np, const int32_t *first, const int32_t *second, double *out,
                            int threads) {
-  (void)n;
   set_threads(threads);
 #pragma omp parallel for schedule(dynamic, 256)
   for (int64_t i = 0; i < np; i++) {
     int32_t a = first[i], b = second[i];
+    if (a < 0 || a >= n || b < 0 || b >= n) { /* ids outside the graph: NaN */
+      out[i] = NAN;
+      continue;
+    }
     double Sa = 0.0, Sb = 0.0;
     for (int32_t k = off[a]; k < off[a + 1]; k++) Sa += v[ind[k]];
     for (int32_t k = off[b]; k < off[b + 1]; k++) Sb += v[ind[k]];

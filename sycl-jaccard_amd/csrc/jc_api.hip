@@ -127,6 +127,11 @@ int run_jaccard(const jc_csr* g, const float* vw, float* result, bool ec, unsign
 
 }  // namespace
 
+namespace jc {
+// Lets the other translation units report a failing HIP call through jc_last_hip_error().
+int note_hip_error(hipError_t e) { return hip_fail(e); }
+}  // namespace jc
+
 extern "C" {
 
 size_t jc_jaccard_workspace_size(int32_t n, int32_t m, int weighted, int edge_centric) {

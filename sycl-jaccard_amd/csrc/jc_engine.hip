@@ -825,13 +825,26 @@ struct Sched {
 
 __device__ __forceinline__ int bucket_of(int d) { return 31 - __clz(d); }
 
+// Virtual row v -> graph row: identity without segments, else segment lookup
+// (segs[i].y = rows before segment i, segs[nseg].y = total).
+__device__ __forceinline__ int32_t seg_row(const int2* __restrict__ segs, int nseg, int32_t v) {
+  if (nseg == 0) return v;
+  int lo = 0, hi = nseg;  // segs[lo].y <= v < segs[hi].y
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (segs[mid].y <= v) lo = mid; else hi = mid;
+  }
+  return segs[lo].x + (v - segs[lo].y);
+}
+
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_sched_count(int32_t n, const int32_t* __restrict__ off,
-                                                       Sched* sc, int small_max) {
+__global__ __launch_bounds__(BLOCK) void k_sched_count(int32_t rows, const int32_t* __restrict__ off,
+                                                       Sched* sc, int small_max, const int2* segs, int nseg) {
   __shared__ int hist[kBuckets + 1];
   for (int i = threadIdx.x; i < kBuckets + 1; i += BLOCK) hist[i] = 0;
   __syncthreads();
-  for (int32_t u = blockIdx.x * BLOCK + threadIdx.x; u < n; u += gridDim.x * BLOCK) {
+  for (int32_t v = blockIdx.x * BLOCK + threadIdx.x; v < rows; v += gridDim.x * BLOCK) {
+    const int32_t u = seg_row(segs, nseg, v);
     const int d = off[u + 1] - off[u];
     if (d <= 0) continue;
     if (d <= small_max)
@@ -863,16 +876,18 @@ __global__ void k_sched_scan(Sched* sc) {
 }
 
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_sched_emit(int32_t n, const int32_t* __restrict__ off,
-                                                      Sched* sc, int2* tasks, int32_t* small_rows, int small_max) {
+__global__ __launch_bounds__(BLOCK) void k_sched_emit(int32_t rows, const int32_t* __restrict__ off,
+                                                      Sched* sc, int2* tasks, int32_t* small_rows, int small_max,
+                                                      const int2* segs, int nseg) {
   __shared__ int hist[kBuckets + 1];
   __shared__ int basep[kBuckets + 1];
-  for (int32_t u0 = blockIdx.x * BLOCK; u0 < n; u0 += gridDim.x * BLOCK) {
+  for (int32_t v0 = blockIdx.x * BLOCK; v0 < rows; v0 += gridDim.x * BLOCK) {
     for (int i = threadIdx.x; i < kBuckets + 1; i += BLOCK) hist[i] = 0;
     __syncthreads();
-    const int32_t u = u0 + threadIdx.x;
+    const int32_t v = v0 + threadIdx.x;
+    const int32_t u = v < rows ? seg_row(segs, nseg, v) : 0;
     int d = 0, b = -1, cntv = 0, local = 0;
-    if (u < n) {
+    if (v < rows) {
       d = off[u + 1] - off[u];
       if (d > 0) {
         if (d <= small_max) {
@@ -923,13 +938,16 @@ static hipError_t run_engine_t(const EngineArgs& base, void* ws, bool fixup, hip
   constexpr int BB = JC_BLOCK;  // block kernel block
   constexpr int PB = 256;   // schedule kernels
   EngineArgs a = base;
+  if (a.nseg > 0 && fixup) return hipErrorInvalidValue;  // segments need a symmetric graph
+  const int32_t rows = a.nseg > 0 ? a.seg_rows : a.n;
+  const int64_t edges = a.nseg > 0 ? a.seg_edges : a.m;
   if (a.small_max < 1 || a.small_max > kSmallMax) a.small_max = kSmallMax;
   if (a.small_blocks_per_cu < 1) a.small_blocks_per_cu = 16;
   unsigned char* p = static_cast<unsigned char*>(ws);
   Sched* sc = reinterpret_cast<Sched*>(p);
   p += align_up(sizeof(Sched), 256);
   int2* tasks = reinterpret_cast<int2*>(p);
-  p += align_up((size_t)max_tasks(a.n, a.m) * sizeof(int2), 256);
+  p += align_up((size_t)max_tasks(rows, edges) * sizeof(int2), 256);
   int32_t* small = reinterpret_cast<int32_t*>(p);
   a.tasks = tasks;
   a.num_tasks = &sc->num_tasks;
@@ -941,13 +959,15 @@ static hipError_t run_engine_t(const EngineArgs& base, void* ws, bool fixup, hip
   hipError_t err;
   hipLaunchKernelGGL(k_sched_init, dim3(1), dim3(128), 0, stream, sc);
   if ((err = hipGetLastError()) != hipSuccess) return err;
-  if (a.n > 0) {
-    const int g = std::min<int64_t>((a.n + PB - 1) / PB, (int64_t)cus * 8);
+  if (rows > 0) {
+    const int g = std::min<int64_t>((rows + PB - 1) / PB, (int64_t)cus * 8);
     {
       ProfScope ps("sched", stream);
-      hipLaunchKernelGGL(k_sched_count<PB>, dim3(g), dim3(PB), 0, stream, a.n, a.off, sc, a.small_max);
+      hipLaunchKernelGGL(k_sched_count<PB>, dim3(g), dim3(PB), 0, stream, rows, a.off, sc, a.small_max, a.segs,
+                         a.nseg);
       hipLaunchKernelGGL(k_sched_scan, dim3(1), dim3(64), 0, stream, sc);
-      hipLaunchKernelGGL(k_sched_emit<PB>, dim3(g), dim3(PB), 0, stream, a.n, a.off, sc, tasks, small, a.small_max);
+      hipLaunchKernelGGL(k_sched_emit<PB>, dim3(g), dim3(PB), 0, stream, rows, a.off, sc, tasks, small, a.small_max,
+                         a.segs, a.nseg);
       if ((err = hipGetLastError()) != hipSuccess) return err;
     }
 
@@ -964,7 +984,7 @@ static hipError_t run_engine_t(const EngineArgs& base, void* ws, bool fixup, hip
       hipLaunchKernelGGL((k_owner_block<MODE, W, BB>), dim3(cus * occ), dim3(BB), Cfg::lds_bytes, stream, a);
       if ((err = hipGetLastError()) != hipSuccess) return err;
     }
-    const int sg = std::min<int64_t>((a.n + (SB / kWave) - 1) / (SB / kWave), (int64_t)cus * a.small_blocks_per_cu);
+    const int sg = std::min<int64_t>((rows + (SB / kWave) - 1) / (SB / kWave), (int64_t)cus * a.small_blocks_per_cu);
     {
       ProfScope ps("owner_small", stream);
       hipLaunchKernelGGL((k_owner_small<MODE, W, SB>), dim3(sg), dim3(SB), 0, stream, a);

@@ -46,10 +46,19 @@ struct EngineArgs {
   // tuning (results do not depend on these)
   int small_max;              // degree bin threshold: one wave per row up to here (1..kSmallMax)
   int small_blocks_per_cu;    // grid of the one-wave-per-row kernel
+  // Row segments (device memory, optional): when nseg > 0 only the rows of
+  // the segments are computed. segs[i] = (first row, running row count before
+  // segment i) for i < nseg, and segs[nseg] = (0, total rows); seg_rows and
+  // seg_edges are the totals (rows and stored edges of all segments).
+  const int2* segs;
+  int nseg;
+  int32_t seg_rows;
+  int64_t seg_edges;
 };
 
 // Workspace needed by an owner-engine pass over a graph with n vertices and m
-// stored edges (task list, small-row list, counters).
+// stored edges (task list, small-row list, counters). With row segments, n and
+// m are the segments' totals.
 size_t engine_workspace_bytes(int32_t n, int64_t m);
 
 // Owner engine: every pair written once per stored direction. When
@@ -64,6 +73,9 @@ hipError_t run_pairs(int mode, bool weighted, int32_t n, const int32_t* off, con
                      const float* vw, const float* work32, int64_t np, const int32_t* first,
                      const int32_t* second, float* out32, float* ws32, int accumulate,
                      hipStream_t stream);
+
+// Records a failing HIP call for jc_last_hip_error(); returns the JC_ERR_* code.
+int note_hip_error(hipError_t e);
 
 // Streaming kernels.
 hipError_t launch_fill_f32(float* x, int64_t n, float v, hipStream_t s);

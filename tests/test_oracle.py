@@ -130,3 +130,13 @@ def test_nonzero_count_threshold():
 def test_oracle_thread_count_independent():
     g = graphs.rmat(10, 16, seed=9)
     assert np.array_equal(bits(oracle.ec_unweighted(g, threads=1)), bits(oracle.ec_unweighted(g, threads=4)))
+
+
+def test_pairs_outside_graph_are_nan():
+    g = cases.two_triangles()
+    f = np.array([0, -1, 2, g.n], np.int32)
+    s = np.array([1, 0, g.n, 3], np.int32)
+    pu = oracle.pairs_unweighted(g, f, s)
+    assert pu[0] == 0.25 and np.isnan(pu[1:]).all()
+    pw = oracle.pairs_weighted_f64(g, np.ones(g.n), f, s)
+    assert pw[0] == 0.25 and np.isnan(pw[1:]).all()

@@ -48,7 +48,7 @@ bool csr_args_ok(const jc_csr* g) {
 
 // Layout of the jc_jaccard_ex workspace.
 struct Layout {
-  size_t engine, rows, s64, work32, wi, ws, total;
+  size_t engine, rows, s64, exact, work32, wi, ws, total;
 };
 
 Layout layout(int32_t n, int32_t m, bool weighted, bool ec) {
@@ -60,6 +60,8 @@ Layout layout(int32_t n, int32_t m, bool weighted, bool ec) {
   if (weighted) o += al(jc::rowsum_scratch_bytes(n));
   L.s64 = o;
   if (weighted) o += al((size_t)std::max(n, 1) * sizeof(double));
+  L.exact = o;
+  if (weighted) o += al(sizeof(jc::ExactInfo));
   L.work32 = o;
   if (!weighted && !ec) o += al((size_t)std::max(n, 1) * sizeof(float));
   L.wi = o;
@@ -93,7 +95,12 @@ int run_jaccard(const jc_csr* g, const float* vw, float* result, bool ec, unsign
   a.small_max = tuning ? tuning->small_max : 0;
   a.small_blocks_per_cu = tuning ? tuning->small_blocks_per_cu : 0;
   double* S64 = weighted ? reinterpret_cast<double*>(ws + L.s64) : nullptr;
-  if (weighted) JC_TRY(jc::launch_rowsum_f64(g->n, g->offsets, g->indices, vw, S64, ws + L.rows, s));
+  if (weighted) {
+    jc::ExactInfo* ex = reinterpret_cast<jc::ExactInfo*>(ws + L.exact);
+    JC_TRY(jc::launch_rowsum_f64(g->n, g->offsets, g->indices, vw, S64, ws + L.rows, s));
+    JC_TRY(jc::launch_exact_info(g->n, vw, S64, ex, s));
+    a.exact = ex;
+  }
   a.S64 = S64;
   if (ec) {
     a.out32 = result;

@@ -393,6 +393,52 @@ hipError_t launch_symcheck(int32_t n, int64_t m, const int32_t* off, const int32
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Range facts of vertex weights and row sums (ExactInfo): smallest exponent
+// of a set bit over all weights, bound exponent of the largest row sum, and
+// whether any weight or sum is negative or not finite.
+__global__ void k_exact_init(ExactInfo* e) {
+  if (threadIdx.x == 0) *e = ExactInfo{INT32_MAX, INT32_MIN, 0, 0};
+}
+
+__global__ __launch_bounds__(256) void k_exact_scan(int32_t n, const float* __restrict__ vw,
+                                                    const double* __restrict__ S, ExactInfo* info) {
+  int lsb = INT32_MAX, mx = INT32_MIN, bad = 0;
+  for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t b = __float_as_uint(vw[i]);
+    const uint32_t ex = (b >> 23) & 0xFFu, man = b & 0x7FFFFFu;
+    if ((b >> 31) && (b << 1)) bad = 1;  // negative (-0 is harmless)
+    if (ex == 0xFFu) {
+      bad = 1;
+    } else if (ex != 0 || man != 0) {
+      const uint32_t full = ex ? (man | 0x800000u) : man;
+      lsb = min(lsb, (ex ? (int)ex - 150 : -149) + __builtin_ctz(full));
+    }
+    const double sv = S[i];
+    if (!(sv >= 0.0) || isinf(sv))
+      bad = 1;
+    else if (sv > 0.0)
+      mx = max(mx, ilogb(sv) + 1);
+  }
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    lsb = min(lsb, __shfl_xor(lsb, o));
+    mx = max(mx, __shfl_xor(mx, o));
+    bad |= __shfl_xor(bad, o);
+  }
+  if (lane_id() == 0) {
+    if (lsb != INT32_MAX) atomicMin(&info->min_lsb, lsb);
+    if (mx != INT32_MIN) atomicMax(&info->max_exp, mx);
+    if (bad) atomicOr(&info->bad, 1);
+  }
+}
+
+hipError_t launch_exact_info(int32_t n, const float* vw, const double* S, ExactInfo* info, hipStream_t s) {
+  hipLaunchKernelGGL(k_exact_init, dim3(1), dim3(64), 0, s, info);
+  if (n > 0) hipLaunchKernelGGL(k_exact_scan, dim3(grid_for(n, 256 * 4, 4)), dim3(256), 0, s, n, vw, S, info);
+  return hipGetLastError();
+}
+
 int device_cu_count() {
   static int cached[64] = {0};
   int dev = 0;

@@ -35,6 +35,7 @@
 //   * Positions that no symmetric pair reaches (asymmetric CSR input) keep a
 //     sentinel and are recomputed by a fix-up pass with the pair routine.
 #include "jc_device.hpp"
+#include "jc_hash.hpp"
 #include "jc_internal.hpp"
 #include "jc_profile.hpp"
 
@@ -74,6 +75,16 @@ __device__ __forceinline__ DD wave_sum_dd(DD v) {
 
 __device__ __forceinline__ void lds_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup"); }
 
+// True when every weight sum of this pass is exact in plain fp64: all vertex
+// weights non-negative multiples of 2^min_lsb and every row sum below
+// 2^max_exp with max_exp - min_lsb <= 52 (then every partial sum of a subset
+// of a row fits 53 bits). Double-double and plain fp64 then give the same bits.
+__device__ __forceinline__ bool exact_f64(const EngineArgs& a) {
+  if (a.exact == nullptr) return false;
+  const ExactInfo e = *a.exact;
+  return e.bad == 0 && e.max_exp - e.min_lsb <= 52;
+}
+
 // ---------------------------------------------------------------------------
 // Bucketized LDS hash table: 2^lb buckets of kBucket int32 slots. Slots 0..6
 // hold keys (-1 = empty); slot 7 is the spill marker, set when a key had to
@@ -83,6 +94,9 @@ __device__ __forceinline__ void lds_fence() { __builtin_amdgcn_fence(__ATOMIC_SE
 // multiplicative hash.
 #ifndef JC_WVAL_SAMEPASS
 #define JC_WVAL_SAMEPASS 0
+#endif
+#ifndef JC_F64FAST
+#define JC_F64FAST 1
 #endif
 constexpr int kBucket = 8;
 constexpr int kBucketKeys = 7;
@@ -98,9 +112,9 @@ template <bool W>
 struct Table {
   int32_t* key;
   float* val;
-  int shift;        // bucket = (key * C) >> shift
+  int lb;           // log2 of the bucket count
   uint32_t bmask;   // buckets - 1
-  __device__ __forceinline__ uint32_t bucket(int32_t y) const { return ((uint32_t)y * 0x9E3779B1u) >> shift; }
+  __device__ __forceinline__ uint32_t bucket(int32_t y) const { return hash_bucket(y, lb); }
   __device__ __forceinline__ uint32_t next(uint32_t b) const { return (b + 1) & bmask; }
   __device__ __forceinline__ void clear(int i) const { key[i] = -1; }
   __device__ __forceinline__ void insert(int32_t y, float v) const {
@@ -300,8 +314,11 @@ struct Chunk {
   DD* pacc;
 };
 
-// Probes U elements per lane: per-lane hit count and weight sum.
-template <bool W, int U>
+// Probes U elements per lane: per-lane hit count and weight sum. FAST: the
+// weight sums are known to be exact in plain fp64 (see ExactInfo), so hits
+// are added with one fp64 add instead of a double-double add; the result is
+// the same.
+template <bool W, int U, bool FAST>
 __device__ __forceinline__ void probe_u(const Chunk<W>& cx, const int32_t (&y)[U], int& cnt, DD& acc) {
   uint32_t bk[U];
   Hit h[U];
@@ -324,8 +341,72 @@ __device__ __forceinline__ void probe_u(const Chunk<W>& cx, const int32_t (&y)[U
 #pragma unroll
   for (int t = 0; t < U; ++t) {
     cnt += h[t].hit ? 1 : 0;
-    if constexpr (W) acc = dd_add1(acc, (double)h[t].v);
+    if constexpr (W) {
+      if constexpr (FAST)
+        acc.hi += (double)h[t].v;
+      else
+        acc = dd_add1(acc, (double)h[t].v);
+    }
   }
+}
+
+// Streams entries [s, e) of a short row (bounds uniform in the group of G
+// lanes), U loads per lane per pass; the next pass's loads are issued before
+// the current pass is probed, so global latency overlaps the LDS probes.
+// COUNT: also count the entries below u and whether u occurs (mirror
+// position). STOP: the owner's chunk ends at id cx.hi; stop after the pass
+// that crosses it and return the position of the first entry above cx.hi
+// (entries of that pass beyond it are probed too, and miss). Returns the
+// stop position, or e.
+template <bool W, int G, int U, bool FAST, bool COUNT, bool STOP>
+__device__ __forceinline__ int stream(const EngineArgs& a, const Chunk<W>& cx, int s, int e, int& cnt, DD& acc,
+                                      int& below, int& found) {
+  const int gl = lane_id() & (G - 1);
+  int32_t y[U];
+  int p0 = s;
+  if (p0 < e) {
+#pragma unroll
+    for (int t = 0; t < U; ++t) y[t] = a.ind[min(p0 + t * G + gl, e - 1)];
+  }
+  while (p0 < e) {
+    const int pn = p0 + G * U;
+    int32_t yn[U];
+    if (pn < e) {
+#pragma unroll
+      for (int t = 0; t < U; ++t) yn[t] = a.ind[min(pn + t * G + gl, e - 1)];
+    }
+    if (pn > e) {  // partial last pass: clamped lanes probe a key that is never stored
+#pragma unroll
+      for (int t = 0; t < U; ++t)
+        if (p0 + t * G + gl >= e) y[t] = INT32_MAX;
+    }
+    if constexpr (COUNT) {
+#pragma unroll
+      for (int t = 0; t < U; ++t) {
+        below += y[t] < cx.u ? 1 : 0;
+        found |= y[t] == cx.u ? 1 : 0;
+      }
+    }
+    probe_u<W, U, FAST>(cx, y, cnt, acc);
+    if constexpr (STOP) {
+      const uint64_t gm = group_mask<G>();
+      // rows are sorted: the pass crosses the bound iff its last entry does
+      if (ballot(y[U - 1] > cx.hi) & gm) {
+        const int gb = lane_id() & ~(G - 1);
+        int first = e;
+#pragma unroll
+        for (int t = U - 1; t >= 0; --t) {
+          const uint64_t bm = ballot(y[t] > cx.hi) & gm;
+          if (bm) first = p0 + t * G + (__ffsll((unsigned long long)bm) - 1 - gb);
+        }
+        return min(first, e);
+      }
+    }
+    p0 = pn;
+#pragma unroll
+    for (int t = 0; t < U; ++t) y[t] = yn[t];
+  }
+  return e;
 }
 
 // One unit of work: each group of G lanes handles one owned short row x
@@ -336,7 +417,7 @@ __device__ __forceinline__ void probe_u(const Chunk<W>& cx, const int32_t (&y)[U
 //     chunk's id bound, store the partial and the resume position; on the
 //     last chunk stream to the end, add the stored partial, locate u by one
 //     group search and emit.
-template <int MODE, bool W, int G, int U, bool MULTI>
+template <int MODE, bool W, int G, int U, bool MULTI, bool FAST>
 __device__ __forceinline__ void unit(const EngineArgs& a, const Chunk<W>& cx, bool valid, int32_t x,
                                      int ox, int dx, int j, int item) {
   const int gl = lane_id() & (G - 1);
@@ -345,45 +426,19 @@ __device__ __forceinline__ void unit(const EngineArgs& a, const Chunk<W>& cx, bo
   const bool last = !MULTI || cx.c == cx.nch - 1;
   int s = ox;
   if constexpr (MULTI) s = (valid && cx.c > 0) ? cx.cursor[item] : ox;
-  const int e = valid ? end : s;
+  int e = valid ? end : s;
   // rows streamed in one group pass count the entries below u on the way;
   // longer rows locate u with one group search
   constexpr bool kCountMirror = !MULTI && G * U <= 64;
   int cnt = 0, below = 0, found = 0;
   DD acc{0.0, 0.0};
   int stop_at = e;
-  for (int p0 = s; p0 < e; p0 += G * U) {
-    // unconditional loads (clamped index) so all U are in flight together
-    int32_t y[U];
-#pragma unroll
-    for (int t = 0; t < U; ++t) y[t] = a.ind[min(p0 + t * G + gl, e - 1)];
-#pragma unroll
-    for (int t = 0; t < U; ++t) {
-      const int p = p0 + t * G + gl;
-      if (p >= e) y[t] = INT32_MAX;
-      if constexpr (kCountMirror) {
-        below += y[t] < cx.u ? 1 : 0;
-        found |= y[t] == cx.u ? 1 : 0;
-      }
-    }
-    probe_u<W, U>(cx, y, cnt, acc);
-    if constexpr (MULTI) {
-      if (!last) {
-        // entries beyond the chunk's bound belong to later chunks: stop at the first
-        int first = -1;
-#pragma unroll
-        for (int t = 0; t < U; ++t) {
-          const int p = p0 + t * G + gl;
-          const uint64_t bm = ballot(p < e && y[t] > cx.hi) & gm;
-          if (first < 0 && bm) first = p0 + t * G + (__ffsll((unsigned long long)bm) - 1 - (lane_id() & ~(G - 1)));
-        }
-        if (first >= 0) {
-          stop_at = first;
-          break;
-        }
-      }
-    }
-  }
+  if (MULTI && !last)
+    stop_at = stream<W, G, U, FAST, kCountMirror, true>(a, cx, s, e, cnt, acc, below, found);
+  else
+    stream<W, G, U, FAST, kCountMirror, false>(a, cx, s, e, cnt, acc, below, found);
+  (void)gm;
+  (void)gl;
   cnt = grp_sum_i<G>(cnt);
   if constexpr (W) acc = grp_sum_dd<G>(acc);
   if constexpr (!MULTI) {
@@ -443,17 +498,26 @@ __host__ __device__ __forceinline__ constexpr int class_g(int c) {
 
 template <int MODE, bool W, bool MULTI, int C>
 __device__ __forceinline__ void unit_class(const EngineArgs& a, const Chunk<W>& cx, bool valid, int32_t x,
-                                           int ox, int dx, int j, int item) {
+                                           int ox, int dx, int j, int item, bool fast) {
   constexpr int G = class_g(C);
   constexpr int U = C <= 1 ? 1 : C == 2 ? 2 : C == 3 ? 4 : C == 4 ? JC_U4 : JC_U5;
-  unit<MODE, W, G, U, MULTI>(a, cx, valid, x, ox, dx, j, item);
+#if JC_F64FAST
+  if constexpr (W) {
+    if (fast) {
+      unit<MODE, W, G, U, MULTI, true>(a, cx, valid, x, ox, dx, j, item);
+      return;
+    }
+  }
+#endif
+  (void)fast;
+  unit<MODE, W, G, U, MULTI, false>(a, cx, valid, x, ox, dx, j, item);
 }
 
 // Items of one class held in lanes (x, ox, dx, j per lane; mask = owned lanes
 // of the class): processed 64/G at a time, group g taking the g-th item.
 template <int MODE, bool W, int C>
 __device__ __forceinline__ void run_class_regs(const EngineArgs& a, const Chunk<W>& cx, uint64_t mask,
-                                               int32_t x, int ox, int dx, int j) {
+                                               int32_t x, int ox, int dx, int j, bool fast) {
   constexpr int G = class_g(C);
   constexpr int NG = kWave / G;
   const int g = lane_id() / G;
@@ -472,7 +536,7 @@ __device__ __forceinline__ void run_class_regs(const EngineArgs& a, const Chunk<
     const int oxx = __shfl(ox, s);
     const int dxx = __shfl(dx, s);
     const int jj = __shfl(j, s);
-    unit_class<MODE, W, false, C>(a, cx, src >= 0, xx, oxx, dxx, jj, 0);
+    unit_class<MODE, W, false, C>(a, cx, src >= 0, xx, oxx, dxx, jj, 0, fast);
   }
 }
 
@@ -480,18 +544,18 @@ __device__ __forceinline__ void run_class_regs(const EngineArgs& a, const Chunk<
 // run each class (single-chunk owner rows).
 template <int MODE, bool W>
 __device__ __forceinline__ void run_batch_regs(const EngineArgs& a, const Chunk<W>& cx, bool cand, int32_t x,
-                                               int ox, int dx, int j) {
+                                               int ox, int dx, int j, bool fast) {
   const bool own = cand && owns(cx.u, cx.du, x, dx);
   const int c = len_class(dx);
   uint64_t m[kClasses];
 #pragma unroll
   for (int k = 0; k < kClasses; ++k) m[k] = ballot(own && c == k);
-  if (m[5]) run_class_regs<MODE, W, 5>(a, cx, m[5], x, ox, dx, j);
-  if (m[4]) run_class_regs<MODE, W, 4>(a, cx, m[4], x, ox, dx, j);
-  if (m[3]) run_class_regs<MODE, W, 3>(a, cx, m[3], x, ox, dx, j);
-  if (m[2]) run_class_regs<MODE, W, 2>(a, cx, m[2], x, ox, dx, j);
-  if (m[1]) run_class_regs<MODE, W, 1>(a, cx, m[1], x, ox, dx, j);
-  if (m[0]) run_class_regs<MODE, W, 0>(a, cx, m[0], x, ox, dx, j);
+  if (m[5]) run_class_regs<MODE, W, 5>(a, cx, m[5], x, ox, dx, j, fast);
+  if (m[4]) run_class_regs<MODE, W, 4>(a, cx, m[4], x, ox, dx, j, fast);
+  if (m[3]) run_class_regs<MODE, W, 3>(a, cx, m[3], x, ox, dx, j, fast);
+  if (m[2]) run_class_regs<MODE, W, 2>(a, cx, m[2], x, ox, dx, j, fast);
+  if (m[1]) run_class_regs<MODE, W, 1>(a, cx, m[1], x, ox, dx, j, fast);
+  if (m[0]) run_class_regs<MODE, W, 0>(a, cx, m[0], x, ox, dx, j, fast);
 }
 
 // ---------------------------------------------------------------------------
@@ -505,6 +569,7 @@ __global__ __launch_bounds__(BLOCK) void k_owner_small(EngineArgs a) {
   const int w = threadIdx.x / kWave;
   const int lane = lane_id();
   const int nsmall = *a.num_small;
+  const bool fast = exact_f64(a);
   const int stride = gridDim.x * NW;
   for (int r = blockIdx.x * NW + w; r < nsmall; r += stride) {
     const int32_t u = a.small_rows[r];
@@ -526,7 +591,7 @@ __global__ __launch_bounds__(BLOCK) void k_owner_small(EngineArgs a) {
     if ((ballot(c0 && owns(u, du, x0, dx0)) | ballot(c1 && owns(u, du, x1, dx1))) == 0) continue;
     const int lb = table_log_buckets<W>(du);
     Chunk<W> cx;
-    cx.tab = Table<W>{key_all[w], W ? val_all[W ? w : 0] : nullptr, 32 - lb, (uint32_t)((1 << lb) - 1)};
+    cx.tab = Table<W>{key_all[w], W ? val_all[W ? w : 0] : nullptr, lb, (uint32_t)((1 << lb) - 1)};
     for (int i = lane; i < (kBucket << lb); i += kWave) cx.tab.clear(i);
     lds_fence();
     build_table<W>(cx.tab, a.ind, a.vw, ou, du, lane, kWave);
@@ -541,8 +606,8 @@ __global__ __launch_bounds__(BLOCK) void k_owner_small(EngineArgs a) {
     cx.cursor = nullptr;
     cx.pcnt = nullptr;
     cx.pacc = nullptr;
-    run_batch_regs<MODE, W>(a, cx, c0, x0, ox0, dx0, lane);
-    run_batch_regs<MODE, W>(a, cx, c1, x1, ox1, dx1, lane + kWave);
+    run_batch_regs<MODE, W>(a, cx, c0, x0, ox0, dx0, lane, fast);
+    run_batch_regs<MODE, W>(a, cx, c1, x1, ox1, dx1, lane + kWave, fast);
     lds_fence();  // all probes of this row done before the next row clears the table
   }
 }
@@ -578,7 +643,7 @@ struct BlockCfg {
 template <int MODE, bool W, bool MULTI>
 __device__ __forceinline__ void run_unit_lds(const EngineArgs& a, const Chunk<W>& cx, int cls, int first,
                                              int cend, const int32_t* ix, const int32_t* iox,
-                                             const int32_t* idx_, const uint16_t* ij) {
+                                             const int32_t* idx_, const uint16_t* ij, bool fast) {
   const int G = class_g(cls);
   const int item = first + lane_id() / G;
   const bool valid = item < cend;
@@ -588,12 +653,12 @@ __device__ __forceinline__ void run_unit_lds(const EngineArgs& a, const Chunk<W>
   const int dx = idx_[it];
   const int j = ij[it];
   switch (cls) {
-    case 5: unit_class<MODE, W, MULTI, 5>(a, cx, valid, x, ox, dx, j, it); break;
-    case 4: unit_class<MODE, W, MULTI, 4>(a, cx, valid, x, ox, dx, j, it); break;
-    case 3: unit_class<MODE, W, MULTI, 3>(a, cx, valid, x, ox, dx, j, it); break;
-    case 2: unit_class<MODE, W, MULTI, 2>(a, cx, valid, x, ox, dx, j, it); break;
-    case 1: unit_class<MODE, W, MULTI, 1>(a, cx, valid, x, ox, dx, j, it); break;
-    default: unit_class<MODE, W, MULTI, 0>(a, cx, valid, x, ox, dx, j, it); break;
+    case 5: unit_class<MODE, W, MULTI, 5>(a, cx, valid, x, ox, dx, j, it, fast); break;
+    case 4: unit_class<MODE, W, MULTI, 4>(a, cx, valid, x, ox, dx, j, it, fast); break;
+    case 3: unit_class<MODE, W, MULTI, 3>(a, cx, valid, x, ox, dx, j, it, fast); break;
+    case 2: unit_class<MODE, W, MULTI, 2>(a, cx, valid, x, ox, dx, j, it, fast); break;
+    case 1: unit_class<MODE, W, MULTI, 1>(a, cx, valid, x, ox, dx, j, it, fast); break;
+    default: unit_class<MODE, W, MULTI, 0>(a, cx, valid, x, ox, dx, j, it, fast); break;
   }
 }
 
@@ -623,6 +688,7 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
   const int lane = lane_id();
   const int w = tid / kWave;
   const int ntasks = *a.num_tasks;
+  const bool fast = exact_f64(a);
   for (;;) {
     __syncthreads();
     if (tid == 0) s_task = atomicAdd(a.task_counter, 1);
@@ -683,7 +749,7 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
       const int clen = ce - cs;
       const int lb = table_log_buckets<W>(clen);
       Chunk<W> cx;
-      cx.tab = Table<W>{tkey, tval, 32 - lb, (uint32_t)((1 << lb) - 1)};
+      cx.tab = Table<W>{tkey, tval, lb, (uint32_t)((1 << lb) - 1)};
       for (int i = tid; i < (kBucket << lb); i += BLOCK) cx.tab.clear(i);
       if (tid == 0) s_next = 0;
       __syncthreads();
@@ -711,9 +777,9 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
         const int first = s_cbase[k] + (un - s_ubase[k]) * per;
         const int cend = s_cbase[k + 1];
         if (nch == 1)
-          run_unit_lds<MODE, W, false>(a, cx, cls_k, first, cend, ix, iox, idx_, ij);
+          run_unit_lds<MODE, W, false>(a, cx, cls_k, first, cend, ix, iox, idx_, ij, fast);
         else
-          run_unit_lds<MODE, W, true>(a, cx, cls_k, first, cend, ix, iox, idx_, ij);
+          run_unit_lds<MODE, W, true>(a, cx, cls_k, first, cend, ix, iox, idx_, ij, fast);
       }
       __syncthreads();
     }

@@ -24,6 +24,16 @@ enum Mode : int {
 constexpr int kSmallMax = 128;
 constexpr int kTaskSpan = JC_TASK_SPAN;
 
+// Range facts of the vertex weights of a weighted pass (launch_exact_info):
+// every weight is a multiple of 2^min_lsb, every row sum is below 2^max_exp,
+// bad != 0 if a weight is negative or not finite.
+struct ExactInfo {
+  int min_lsb;
+  int max_exp;
+  int bad;
+  int pad;
+};
+
 struct EngineArgs {
   int32_t n;
   int64_t m;
@@ -54,6 +64,9 @@ struct EngineArgs {
   int nseg;
   int32_t seg_rows;
   int64_t seg_edges;
+  // Weighted modes: optional range facts of vw and S64 (device memory); when
+  // they show all sums are exact in fp64 the kernels add in plain fp64.
+  const ExactInfo* exact;
 };
 
 // Workspace needed by an owner-engine pass over a graph with n vertices and m
@@ -73,6 +86,9 @@ hipError_t run_pairs(int mode, bool weighted, int32_t n, const int32_t* off, con
                      const float* vw, const float* work32, int64_t np, const int32_t* first,
                      const int32_t* second, float* out32, float* ws32, int accumulate,
                      hipStream_t stream);
+
+// Fills *info (device memory) for vertex weights vw[n] and their row sums S[n].
+hipError_t launch_exact_info(int32_t n, const float* vw, const double* S, ExactInfo* info, hipStream_t s);
 
 // Records a failing HIP call for jc_last_hip_error(); returns the JC_ERR_* code.
 int note_hip_error(hipError_t e);

@@ -179,6 +179,7 @@ struct jc_server {
   int32_t* ind = nullptr;
   float* W = nullptr;
   double* S64 = nullptr;
+  jc::ExactInfo* exact = nullptr;
   float* out = nullptr;
   size_t arena_bytes = 0;
   std::vector<jc::Member> g;
@@ -211,7 +212,8 @@ void free_server(jc_server* s) {
     (void)hipStreamSynchronize(s->fifo_stream);
     (void)hipStreamDestroy(s->fifo_stream);
   }
-  for (void* p : {(void*)s->off, (void*)s->ind, (void*)s->W, (void*)s->S64, (void*)s->out, (void*)s->pool_mem})
+  for (void* p : {(void*)s->off, (void*)s->ind, (void*)s->W, (void*)s->S64, (void*)s->exact, (void*)s->out,
+                  (void*)s->pool_mem})
     if (p) (void)hipFree(p);
   delete s;
 }
@@ -319,6 +321,7 @@ int jc_server_create(int32_t num_graphs, const jc_csr* graphs, const float* cons
   if (s->weighted) {
     if ((e = hipMalloc(&s->W, (size_t)std::max<int64_t>(N, 1) * sizeof(float))) != hipSuccess) return fail(e);
     if ((e = hipMalloc(&s->S64, (size_t)std::max<int64_t>(N, 1) * sizeof(double))) != hipSuccess) return fail(e);
+    if ((e = hipMalloc(&s->exact, sizeof(jc::ExactInfo))) != hipSuccess) return fail(e);
     s->arena_bytes += (size_t)std::max<int64_t>(N, 1) * (sizeof(float) + sizeof(double));
   }
   hipStream_t st = s->fifo_stream;
@@ -344,6 +347,7 @@ int jc_server_create(int32_t num_graphs, const jc_csr* graphs, const float* cons
   if (s->weighted && N > 0 &&
       (e = jc::launch_rowsum_f64(s->N, s->off, s->ind, s->W, s->S64, nullptr, st)) != hipSuccess)
     return fail(e);
+  if (s->weighted && (e = jc::launch_exact_info(s->N, s->W, s->S64, s->exact, st)) != hipSuccess) return fail(e);
   if ((e = hipStreamSynchronize(st)) != hipSuccess) return fail(e);
 
   if (cfg.mode == JC_SERVE_CONTINUOUS) {
@@ -516,6 +520,7 @@ int jc_server_run(jc_server* s, const jc_request* reqs, int32_t count, jc_reques
       a.ind = s->ind;
       a.vw = s->W;
       a.S64 = s->S64;
+      a.exact = s->exact;
       a.out32 = s->out;
       a.segs = reinterpret_cast<const int2*>(dev + L.segs);
       a.nseg = nseg;

@@ -282,20 +282,80 @@ __device__ __forceinline__ int grp_lower_bound(const int32_t* __restrict__ a, in
   return lo + popc64(ballot(lt) & gm);
 }
 
+// Group sums over G lanes (G = 8, 16, 32, 64), the total in every lane of the
+// group. Butterfly steps within a row of 16 lanes use DPP lane moves (VALU):
+// quad_perm [1,0,3,2] and [2,3,0,1], then row_half_mirror and row_mirror
+// (each pairs every lane with one in the other half of its 8 or 16 lanes).
+// The 16-lane step uses ds_swizzle (xor 16 inside 32 lanes), and the 32-lane
+// step two lane reads. All lanes of the wave must be active.
+#ifndef JC_DPP
+#define JC_DPP 1
+#endif
+template <int CTRL>
+__device__ __forceinline__ int dpp_mov(int v) {
+  return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, false);
+}
+
+template <int STEP>
+__device__ __forceinline__ int xlane_i(int v) {
+#if JC_DPP
+  if constexpr (STEP == 1) return dpp_mov<0xB1>(v);        // quad_perm [1,0,3,2]
+  else if constexpr (STEP == 2) return dpp_mov<0x4E>(v);   // quad_perm [2,3,0,1]
+  else if constexpr (STEP == 4) return dpp_mov<0x141>(v);  // row_half_mirror
+  else if constexpr (STEP == 8) return dpp_mov<0x140>(v);  // row_mirror
+  else return __builtin_amdgcn_ds_swizzle(v, 0x401F);      // xor 16 within 32 lanes
+#else
+  return __shfl_xor(v, STEP);
+#endif
+}
+
+template <int STEP>
+__device__ __forceinline__ double xlane_d(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = xlane_i<STEP>((int)(b & 0xFFFFFFFFll));
+  const int hi = xlane_i<STEP>((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 template <int G>
 __device__ __forceinline__ int grp_sum_i(int v) {
-#pragma unroll
-  for (int s = 1; s < G; s <<= 1) v += __shfl_xor(v, s);
+  v += xlane_i<1>(v);
+  v += xlane_i<2>(v);
+  if constexpr (G >= 8) v += xlane_i<4>(v);
+  if constexpr (G >= 16) v += xlane_i<8>(v);
+  if constexpr (G >= 32) v += xlane_i<16>(v);
+  if constexpr (G >= 64) v = __builtin_amdgcn_readlane(v, 0) + __builtin_amdgcn_readlane(v, 32);
+  return v;
+}
+
+// Plain fp64 group sum (exact when every partial sum is exact).
+template <int G>
+__device__ __forceinline__ double grp_sum_f64(double v) {
+  v += xlane_d<1>(v);
+  v += xlane_d<2>(v);
+  if constexpr (G >= 8) v += xlane_d<4>(v);
+  if constexpr (G >= 16) v += xlane_d<8>(v);
+  if constexpr (G >= 32) v += xlane_d<16>(v);
+  if constexpr (G >= 64) {
+    const long long b = __double_as_longlong(v);
+    const int lo = (int)(b & 0xFFFFFFFFll), hi = (int)(b >> 32);
+    const double a0 = __longlong_as_double(((long long)__builtin_amdgcn_readlane(hi, 0) << 32) |
+                                           (unsigned int)__builtin_amdgcn_readlane(lo, 0));
+    const double a1 = __longlong_as_double(((long long)__builtin_amdgcn_readlane(hi, 32) << 32) |
+                                           (unsigned int)__builtin_amdgcn_readlane(lo, 32));
+    v = a0 + a1;
+  }
   return v;
 }
 
 template <int G>
 __device__ __forceinline__ DD grp_sum_dd(DD v) {
-#pragma unroll
-  for (int s = 1; s < G; s <<= 1) {
-    DD o{__shfl_xor(v.hi, s), __shfl_xor(v.lo, s)};
-    v = dd_add(v, o);
-  }
+  v = dd_add(v, DD{xlane_d<1>(v.hi), xlane_d<1>(v.lo)});
+  v = dd_add(v, DD{xlane_d<2>(v.hi), xlane_d<2>(v.lo)});
+  if constexpr (G >= 8) v = dd_add(v, DD{xlane_d<4>(v.hi), xlane_d<4>(v.lo)});
+  if constexpr (G >= 16) v = dd_add(v, DD{xlane_d<8>(v.hi), xlane_d<8>(v.lo)});
+  if constexpr (G >= 32) v = dd_add(v, DD{xlane_d<16>(v.hi), xlane_d<16>(v.lo)});
+  if constexpr (G >= 64) v = dd_add(v, DD{__shfl_xor(v.hi, 32), __shfl_xor(v.lo, 32)});
   return v;
 }
 
@@ -439,13 +499,24 @@ __device__ __forceinline__ void unit(const EngineArgs& a, const Chunk<W>& cx, bo
     stream<W, G, U, FAST, kCountMirror, false>(a, cx, s, e, cnt, acc, below, found);
   (void)gm;
   (void)gl;
-  cnt = grp_sum_i<G>(cnt);
-  if constexpr (W) acc = grp_sum_dd<G>(acc);
+  if constexpr (kCountMirror) {
+    // one pass of at most U entries per lane: the three counts fit a byte each
+    const int packed = grp_sum_i<G>(cnt | (below << 8) | (found << 16));
+    cnt = packed & 0xFF;
+    below = (packed >> 8) & 0xFF;
+    found = packed >> 16;
+  } else {
+    cnt = grp_sum_i<G>(cnt);
+  }
+  if constexpr (W) {
+    if constexpr (FAST)
+      acc.hi = grp_sum_f64<G>(acc.hi);
+    else
+      acc = grp_sum_dd<G>(acc);
+  }
   if constexpr (!MULTI) {
     int64_t q = -1;
     if constexpr (kCountMirror) {
-      below = grp_sum_i<G>(below);
-      found = grp_sum_i<G>(found);
       if (found) q = (int64_t)ox + below;
     } else {
       int pos = end;
@@ -501,6 +572,9 @@ __device__ __forceinline__ void unit_class(const EngineArgs& a, const Chunk<W>& 
                                            int ox, int dx, int j, int item, bool fast) {
   constexpr int G = class_g(C);
   constexpr int U = C <= 1 ? 1 : C == 2 ? 2 : C == 3 ? 4 : C == 4 ? JC_U4 : JC_U5;
+  // classes 0-3 stream a row (at most 64 entries) in one pass of G * U; the
+  // single-pass mirror count and its byte-packed reduction rely on it
+  static_assert(C > 3 ? G * U > 64 : G * U >= (C == 0 ? 8 : C == 1 ? 16 : C == 2 ? 32 : 64), "class shape");
 #if JC_F64FAST
   if constexpr (W) {
     if (fast) {

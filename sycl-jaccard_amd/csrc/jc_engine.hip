@@ -20,9 +20,10 @@
 //   * A wave is split into groups of G lanes, one short row per group, G and
 //     the unroll U chosen by the row's length class, so short rows do not
 //     leave lanes idle. Counts accumulate per lane (VALU) and are reduced
-//     once per row: scalar instructions stay off the inner loop.
+//     once per row with DPP lane moves; the next pass's loads are issued
+//     before the current pass is probed.
 //   * Degree binning: rows with degree <= kSmallMax run one wave per row with
-//     a per-wave table; longer rows run one workgroup per (row, 512-entry
+//     a per-wave table; longer rows run one workgroup per (row, 896-entry
 //     piece) task from a degree-descending queue (persistent grid, atomic
 //     counter). A task sorts its owned entries into length classes, heaviest
 //     first, and its waves take class units from an LDS counter.
@@ -32,6 +33,8 @@
 //   * Weighted sums are double-double. For fp32 addends whose bit span fits
 //     in 106 bits the sum is exact, so results do not depend on chunking,
 //     task split or lane order: plain, batched and pair paths agree bitwise.
+//     When the weights' range makes plain fp64 sums exact (ExactInfo, checked
+//     per call on the device) the kernels add in fp64, with the same result.
 //   * Positions that no symmetric pair reaches (asymmetric CSR input) keep a
 //     sentinel and are recomputed by a fix-up pass with the pair routine.
 #include "jc_device.hpp"

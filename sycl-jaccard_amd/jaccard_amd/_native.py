@@ -65,6 +65,21 @@ def load_graph_lib() -> ctypes.CDLL:
             lib.jcg_pair_weight.argtypes = [u64, i32, i32]
             lib.jcg_rmat_batch.restype = vp
             lib.jcg_rmat_batch.argtypes = [i32, vp, vp, vp, ctypes.c_int, u64, f64, f64, f64, vp, vp]
+            lib.jcio_load.restype = vp
+            lib.jcio_load.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+            for name, rt in (("jcio_n", i64), ("jcio_m", i64), ("jcio_flags", u64)):
+                getattr(lib, name).restype = rt
+                getattr(lib, name).argtypes = [vp]
+            lib.jcio_offsets.restype = ctypes.POINTER(i32)
+            lib.jcio_offsets.argtypes = [vp]
+            lib.jcio_indices.restype = ctypes.POINTER(i32)
+            lib.jcio_indices.argtypes = [vp]
+            lib.jcio_weights.restype = ctypes.POINTER(ctypes.c_float)
+            lib.jcio_weights.argtypes = [vp]
+            lib.jcio_free.restype = None
+            lib.jcio_free.argtypes = [vp]
+            lib.jcio_save.restype = ctypes.c_int
+            lib.jcio_save.argtypes = [ctypes.c_char_p, i64, i64, vp, vp, vp, u64, ctypes.c_int]
             lib.jcg_stats.restype = None
             lib.jcg_stats.argtypes = [i32, vp, vp, ctypes.POINTER(i64)]
             _graph = lib

@@ -1,0 +1,97 @@
+"""The command-line driver end to end on the GPU: the reference workflow
+(file in, env-selected algorithm, file out, "non zero pairs" line) gives the
+oracle's results."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from jaccard_amd import api, fileio, graphs
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "sycl-jaccard_amd", "bin")
+
+
+def run(*args, env=None):
+    e = {k: v for k, v in os.environ.items() if not k.startswith("JACCARD_")}
+    e.update(env or {})
+    print("[gpu] cli", args, env, flush=True)
+    return subprocess.run([os.path.join(BIN, "jaccard_amd"), *map(str, args)], capture_output=True, text=True,
+                          env=e, timeout=120)
+
+
+def test_ec_mtx_in_mtx_out(tmp_path, cuda):
+    g = graphs.rmat(9, 8, seed=4)
+    src = str(tmp_path / "g.mtx")
+    fileio.save(src, g, values=np.ones(g.m, np.float32), remove_reverse=True)
+    out = tmp_path / "o.mtx"
+    r = run(src, out, env={"JACCARD_FORCE_EDGE_CENTRIC": "1", "JACCARD_FORCE_WEIGHTED": "0",
+                           "JACCARD_IN_CSR_DUMP_FILEPATH": str(tmp_path / "in.csr")})
+    assert r.returncode == 0, r.stderr
+    assert "FORCE Edge-Centric Implementation" in r.stderr
+    assert r.stdout.startswith("Running on ")
+    want = oracle.ec_unweighted(g)
+    m = re.search(r"vertices (\d+)edges (\d+)non zero pairs (\d+)", r.stdout)
+    assert m and int(m.group(1)) == g.n and int(m.group(2)) == g.m
+    assert int(m.group(3)) == oracle.nonzero_count(want)
+    # the dumped input is the symmetrised graph
+    L = fileio.load(str(tmp_path / "in.csr")).graph
+    assert np.array_equal(L.offsets, g.offsets) and np.array_equal(L.indices, g.indices)
+    # output: one direction per pair, 6 significant digits
+    lines = out.read_text().splitlines()
+    assert lines[0] == "%%MatrixMarket matrix coordinate real symmetric"
+    ent = np.array([list(map(float, l.split())) for l in lines[2:]])
+    assert len(ent) == g.m // 2
+    rows = np.repeat(np.arange(g.n), np.diff(g.offsets))
+    key = {(int(a), int(b)): float(v) for a, b, v in zip(rows + 1, g.indices + 1, want)}
+    got = np.array([v for _, _, v in ent])
+    ref = np.array([key[(int(a), int(b))] for a, b, _ in ent])
+    np.testing.assert_allclose(got, ref, rtol=1e-5, atol=1e-7)
+
+
+def test_vc_weighted_csr_matches_library(tmp_path, cuda):
+    g = graphs.rmat(10, 8, seed=5, weighted=True)
+    src = str(tmp_path / "g.csr")
+    fileio.save(src, g)
+    out = str(tmp_path / "o.csr")
+    r = run(src, out, env={"JACCARD_FORCE_VERTEX_CENTRIC": "1", "JACCARD_EVENT_PROFILE": "1",
+                           "JACCARD_OUT_CSR_DUMP_FILEPATH": str(tmp_path / "dump.csr")})
+    assert r.returncode == 0, r.stderr
+    assert "kernel elapsed time" in r.stderr
+    res = fileio.load(out)
+    assert res.weighted
+    dg = api.DeviceCSR.from_host(g, cuda)
+    import torch
+    W = torch.empty(g.n, dtype=torch.float32, device=cuda)
+    api.vertex_weights(dg, out=W)
+    ref = api.jaccard(dg, weights=W, edge_centric=False).cpu().numpy()
+    assert np.array_equal(res.graph.weights.view(np.uint32), ref.view(np.uint32))
+    Wh = W.cpu().numpy().astype(np.float64)
+    want = oracle.weighted_f64(g, Wh)
+    nz = want != 0
+    assert np.max(np.abs(ref[nz] - want[nz]) / want[nz]) <= 1e-6
+    dump = fileio.load(str(tmp_path / "dump.csr")).graph
+    assert np.array_equal(dump.weights.view(np.uint32), ref.view(np.uint32))
+
+
+def test_auto_choice_and_compare(tmp_path, cuda):
+    g = graphs.rmat(8, 8, seed=6)
+    src = str(tmp_path / "g.csr")
+    fileio.save(src, g)
+    r1 = run(src, tmp_path / "ec.csr", env={"JACCARD_FORCE_EDGE_CENTRIC": "1"})
+    r2 = run(src, tmp_path / "auto.csr")
+    r3 = run(src, tmp_path / "vc.mtx", env={"JACCARD_FORCE_EDGE_CENTRIC": "1", "JACCARD_FORCE_VERTEX_CENTRIC": "1"})
+    assert r1.returncode == r2.returncode == r3.returncode == 0
+    assert "AUTO Edge-Centric" in r2.stderr
+    assert "FORCE Vertex-Centric" in r3.stderr and "non zero pairs" not in r3.stdout
+    a = fileio.load(str(tmp_path / "ec.csr")).graph.weights
+    b = fileio.load(str(tmp_path / "auto.csr")).graph.weights
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    cmp = subprocess.run([os.path.join(BIN, "compare_coords"), str(tmp_path / "ec.csr"), str(tmp_path / "auto.csr"), "0"],
+                         capture_output=True, text=True)
+    assert cmp.returncode == 0

@@ -375,7 +375,40 @@ struct Chunk {
   int* cursor;
   uint32_t* pcnt;
   DD* pacc;
+  // bitmap table (rows whose id span fits the LDS): bit (y - lo) of bits is
+  // set for every key y; weighted: rank[w] = keys in the words before w and
+  // bval[r] = weight of the r-th key in id order
+  const uint32_t* bits;
+  const uint32_t* rank;
+  const float* bval;
+  uint32_t span;    // hi - lo + 1
 };
+
+// Probes U elements per lane against a bitmap table: one LDS word per probe;
+// hits of weighted tables read the key's rank and weight.
+template <bool W, int U, bool FAST>
+__device__ __forceinline__ void probe_bm(const Chunk<W>& cx, const int32_t (&y)[U], int& cnt, DD& acc) {
+  uint32_t off[U], wd[U];
+#pragma unroll
+  for (int t = 0; t < U; ++t) {
+    off[t] = (uint32_t)(y[t] - cx.lo);  // below lo wraps to a large value
+    wd[t] = cx.bits[off[t] < cx.span ? off[t] >> 5 : 0u];
+  }
+#pragma unroll
+  for (int t = 0; t < U; ++t) {
+    const uint32_t sh = off[t] & 31u;
+    const bool hit = off[t] < cx.span && ((wd[t] >> sh) & 1u);
+    cnt += hit ? 1 : 0;
+    if constexpr (W) {
+      float v = 0.0f;
+      if (hit) v = cx.bval[cx.rank[off[t] >> 5] + __popc(wd[t] & ((1u << sh) - 1u))];
+      if constexpr (FAST)
+        acc.hi += (double)v;
+      else
+        acc = dd_add1(acc, (double)v);
+    }
+  }
+}
 
 // Probes U elements per lane: per-lane hit count and weight sum. FAST: the
 // weight sums are known to be exact in plain fp64 (see ExactInfo), so hits
@@ -421,7 +454,7 @@ __device__ __forceinline__ void probe_u(const Chunk<W>& cx, const int32_t (&y)[U
 // that crosses it and return the position of the first entry above cx.hi
 // (entries of that pass beyond it are probed too, and miss). Returns the
 // stop position, or e.
-template <bool W, int G, int U, bool FAST, bool COUNT, bool STOP>
+template <bool W, int G, int U, bool FAST, bool COUNT, bool STOP, bool BM>
 __device__ __forceinline__ int stream(const EngineArgs& a, const Chunk<W>& cx, int s, int e, int& cnt, DD& acc,
                                       int& below, int& found) {
   const int gl = lane_id() & (G - 1);
@@ -450,7 +483,10 @@ __device__ __forceinline__ int stream(const EngineArgs& a, const Chunk<W>& cx, i
         found |= y[t] == cx.u ? 1 : 0;
       }
     }
-    probe_u<W, U, FAST>(cx, y, cnt, acc);
+    if constexpr (BM)
+      probe_bm<W, U, FAST>(cx, y, cnt, acc);
+    else
+      probe_u<W, U, FAST>(cx, y, cnt, acc);
     if constexpr (STOP) {
       const uint64_t gm = group_mask<G>();
       // rows are sorted: the pass crosses the bound iff its last entry does
@@ -480,7 +516,7 @@ __device__ __forceinline__ int stream(const EngineArgs& a, const Chunk<W>& cx, i
 //     chunk's id bound, store the partial and the resume position; on the
 //     last chunk stream to the end, add the stored partial, locate u by one
 //     group search and emit.
-template <int MODE, bool W, int G, int U, bool MULTI, bool FAST>
+template <int MODE, bool W, int G, int U, bool MULTI, bool FAST, bool BM>
 __device__ __forceinline__ void unit(const EngineArgs& a, const Chunk<W>& cx, bool valid, int32_t x,
                                      int ox, int dx, int j, int item) {
   const int gl = lane_id() & (G - 1);
@@ -497,9 +533,9 @@ __device__ __forceinline__ void unit(const EngineArgs& a, const Chunk<W>& cx, bo
   DD acc{0.0, 0.0};
   int stop_at = e;
   if (MULTI && !last)
-    stop_at = stream<W, G, U, FAST, kCountMirror, true>(a, cx, s, e, cnt, acc, below, found);
+    stop_at = stream<W, G, U, FAST, kCountMirror, true, BM>(a, cx, s, e, cnt, acc, below, found);
   else
-    stream<W, G, U, FAST, kCountMirror, false>(a, cx, s, e, cnt, acc, below, found);
+    stream<W, G, U, FAST, kCountMirror, false, BM>(a, cx, s, e, cnt, acc, below, found);
   (void)gm;
   (void)gl;
   if constexpr (kCountMirror) {
@@ -570,7 +606,7 @@ __host__ __device__ __forceinline__ constexpr int class_g(int c) {
   return c == 0 ? 8 : c <= 3 ? 16 : c == 4 ? 32 : 64;
 }
 
-template <int MODE, bool W, bool MULTI, int C>
+template <int MODE, bool W, bool MULTI, int C, bool BM = false>
 __device__ __forceinline__ void unit_class(const EngineArgs& a, const Chunk<W>& cx, bool valid, int32_t x,
                                            int ox, int dx, int j, int item, bool fast) {
   constexpr int G = class_g(C);
@@ -581,13 +617,13 @@ __device__ __forceinline__ void unit_class(const EngineArgs& a, const Chunk<W>& 
 #if JC_F64FAST
   if constexpr (W) {
     if (fast) {
-      unit<MODE, W, G, U, MULTI, true>(a, cx, valid, x, ox, dx, j, item);
+      unit<MODE, W, G, U, MULTI, true, BM>(a, cx, valid, x, ox, dx, j, item);
       return;
     }
   }
 #endif
   (void)fast;
-  unit<MODE, W, G, U, MULTI, false>(a, cx, valid, x, ox, dx, j, item);
+  unit<MODE, W, G, U, MULTI, false, BM>(a, cx, valid, x, ox, dx, j, item);
 }
 
 // Items of one class held in lanes (x, ox, dx, j per lane; mask = owned lanes
@@ -703,6 +739,27 @@ __global__ __launch_bounds__(BLOCK) void k_owner_small(EngineArgs a) {
 #endif
 constexpr size_t kBlockLds = JC_BLOCK_LDS_KIB * 1024 - 512;  // minus the kernel's static shared variables
 
+#ifndef JC_BITMAP
+#define JC_BITMAP 1
+#endif
+
+// Exclusive prefix sum over the workgroup (all threads active).
+__device__ __forceinline__ int block_excl_scan(int v, int* s_wsum) {
+  const int lane = lane_id(), w = threadIdx.x / kWave;
+  int incl = v;
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const int t = __shfl_up(incl, o);
+    if (lane >= o) incl += t;
+  }
+  if (lane == kWave - 1) s_wsum[w] = incl;
+  __syncthreads();
+  int base = 0;
+  for (int i = 0; i < w; ++i) base += s_wsum[i];
+  __syncthreads();
+  return base + incl - v;
+}
+
 template <bool W>
 struct BlockCfg {
   static constexpr size_t item_bytes = (size_t)kTaskSpan * (3 * sizeof(int32_t) + sizeof(uint16_t));
@@ -717,7 +774,7 @@ struct BlockCfg {
   static constexpr size_t lds_bytes = key_bytes + val_bytes + item_bytes + cursor_bytes + part_bytes;
 };
 
-template <int MODE, bool W, bool MULTI>
+template <int MODE, bool W, bool MULTI, bool BM = false>
 __device__ __forceinline__ void run_unit_lds(const EngineArgs& a, const Chunk<W>& cx, int cls, int first,
                                              int cend, const int32_t* ix, const int32_t* iox,
                                              const int32_t* idx_, const uint16_t* ij, bool fast) {
@@ -730,12 +787,12 @@ __device__ __forceinline__ void run_unit_lds(const EngineArgs& a, const Chunk<W>
   const int dx = idx_[it];
   const int j = ij[it];
   switch (cls) {
-    case 5: unit_class<MODE, W, MULTI, 5>(a, cx, valid, x, ox, dx, j, it, fast); break;
-    case 4: unit_class<MODE, W, MULTI, 4>(a, cx, valid, x, ox, dx, j, it, fast); break;
-    case 3: unit_class<MODE, W, MULTI, 3>(a, cx, valid, x, ox, dx, j, it, fast); break;
-    case 2: unit_class<MODE, W, MULTI, 2>(a, cx, valid, x, ox, dx, j, it, fast); break;
-    case 1: unit_class<MODE, W, MULTI, 1>(a, cx, valid, x, ox, dx, j, it, fast); break;
-    default: unit_class<MODE, W, MULTI, 0>(a, cx, valid, x, ox, dx, j, it, fast); break;
+    case 5: unit_class<MODE, W, MULTI, 5, BM>(a, cx, valid, x, ox, dx, j, it, fast); break;
+    case 4: unit_class<MODE, W, MULTI, 4, BM>(a, cx, valid, x, ox, dx, j, it, fast); break;
+    case 3: unit_class<MODE, W, MULTI, 3, BM>(a, cx, valid, x, ox, dx, j, it, fast); break;
+    case 2: unit_class<MODE, W, MULTI, 2, BM>(a, cx, valid, x, ox, dx, j, it, fast); break;
+    case 1: unit_class<MODE, W, MULTI, 1, BM>(a, cx, valid, x, ox, dx, j, it, fast); break;
+    default: unit_class<MODE, W, MULTI, 0, BM>(a, cx, valid, x, ox, dx, j, it, fast); break;
   }
 }
 
@@ -761,6 +818,7 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
   __shared__ int s_wcnt[kClasses][NW];
   __shared__ int s_cbase[kClasses + 1];   // item ranges per class, heaviest first
   __shared__ int s_ubase[kClasses + 1];   // unit ranges per class
+  __shared__ int s_wsum[NW];
   const int tid = threadIdx.x;
   const int lane = lane_id();
   const int w = tid / kWave;
@@ -778,7 +836,12 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
     const int du = a.off[u + 1] - ou;
     const int ka = ou + tk.y * kTaskSpan;
     const int span = min(kTaskSpan, ou + du - ka);
-    const int nch = (du + CH - 1) / CH;
+    // bitmap table when the row's id span fits the table region of the LDS
+    const int32_t lo0 = a.ind[ou], hi0 = a.ind[ou + du - 1];
+    const uint32_t words = ((uint32_t)(hi0 - lo0) >> 5) + 1u;
+    const bool bm = JC_BITMAP &&
+                    (size_t)words * (W ? 8u : 4u) + (W ? (size_t)4 * du : 0) <= Cfg::key_bytes + Cfg::val_bytes;
+    const int nch = bm ? 1 : (du + CH - 1) / CH;
 
     // owned entries of the task, sorted into length classes (heaviest first)
     int32_t x = 0;
@@ -820,6 +883,63 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
       ij[at] = (uint16_t)tid;
     }
     const int nunits = s_ubase[kClasses];
+    if (bm) {
+      // bits, then (weighted) the rank of every word and the keys' weights
+      uint32_t* bits = reinterpret_cast<uint32_t*>(smem);
+      uint32_t* brank = bits + words;
+      float* bval = reinterpret_cast<float*>(brank + (W ? words : 0u));
+      for (uint32_t i = tid; i < words; i += BLOCK) bits[i] = 0u;
+      if (tid == 0) s_next = 0;
+      __syncthreads();
+      for (int i = tid; i < du; i += BLOCK) {
+        const int32_t y = a.ind[ou + i];
+        const uint32_t o = (uint32_t)(y - lo0);
+        if (o < words * 32u) atomicOr(&bits[o >> 5], 1u << (o & 31u));  // rows are sorted: always true
+        if constexpr (W) bval[i] = a.vw[y];  // the i-th key in id order has rank i
+      }
+      __syncthreads();
+      if constexpr (W) {
+        const uint32_t per = (words + BLOCK - 1) / BLOCK;
+        const uint32_t w0 = min(words, (uint32_t)tid * per), w1 = min(words, w0 + per);
+        int c = 0;
+        for (uint32_t i = w0; i < w1; ++i) c += __popc(bits[i]);
+        int r = block_excl_scan(c, s_wsum);
+        for (uint32_t i = w0; i < w1; ++i) {
+          brank[i] = (uint32_t)r;
+          r += __popc(bits[i]);
+        }
+        __syncthreads();
+      }
+      Chunk<W> cx;
+      cx.u = u;
+      cx.du = du;
+      cx.lo = lo0;
+      cx.hi = hi0;
+      cx.c = 0;
+      cx.nch = 1;
+      cx.kbase = ka;
+      cx.cursor = cursor;
+      cx.pcnt = pcnt;
+      cx.pacc = pacc;
+      cx.bits = bits;
+      cx.rank = brank;
+      cx.bval = bval;
+      cx.span = (uint32_t)(hi0 - lo0) + 1u;
+      for (;;) {
+        int un = 0;
+        if (lane == 0) un = atomicAdd(&s_next, 1);
+        un = __shfl(un, 0);
+        if (un >= nunits) break;
+        int k = 0;
+        while (un >= s_ubase[k + 1]) ++k;
+        const int cls_k = kClasses - 1 - k;
+        const int per = kWave / class_g(cls_k);
+        const int first = s_cbase[k] + (un - s_ubase[k]) * per;
+        run_unit_lds<MODE, W, false, true>(a, cx, cls_k, first, s_cbase[k + 1], ix, iox, idx_, ij, fast);
+      }
+      __syncthreads();
+      continue;
+    }
     for (int c = 0; c < nch; ++c) {
       const int cs = ou + c * CH;
       const int ce = min(cs + CH, ou + du);

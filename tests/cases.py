@@ -48,6 +48,17 @@ def hub_graph(n=30000, extra=8, seed=3):
     return graphs.from_edges(n, np.concatenate(src), np.concatenate(dst), weighted=True)
 
 
+def sparse_hub_graph(n=1 << 21, k=30000, extra=8, seed=4):
+    """Vertex 0 adjacent to k vertices spread over 2^21 ids, plus random edges
+    among them: the hub's id span is too wide for an LDS bitmap, so its row
+    runs through several hash-table chunks."""
+    rng = np.random.default_rng(seed)
+    nb = rng.choice(np.arange(1, n), size=k, replace=False)
+    src = [np.zeros(k, np.int64), nb[rng.integers(0, k, size=k * extra // 2)]]
+    dst = [nb, nb[rng.integers(0, k, size=k * extra // 2)]]
+    return graphs.from_edges(n, np.concatenate(src), np.concatenate(dst), weighted=True)
+
+
 def random_directed(n=2000, m=20000, seed=5):
     """An asymmetric CSR (no reverse edges added): exercises the fix-up pass."""
     rng = np.random.default_rng(seed)

@@ -70,10 +70,12 @@ def test_weighted_vs_fp64(name, cuda):
     assert_bits_equal(ec, vc, f"{name} weighted EC vs VC")
 
 
-def test_hub_multichunk(cuda):
-    """A row of degree 29999 spans several LDS chunks in both modes."""
-    g = cases.hub_graph()
-    print(f"[gpu] hub n={g.n} m={g.m}", flush=True)
+@pytest.mark.parametrize("kind", ["dense_span", "wide_span"])
+def test_hub_multichunk(kind, cuda):
+    """A hub row of degree ~30000: with ids 0..29999 it fits one LDS bitmap;
+    spread over 2^21 ids it spans several hash-table chunks in both modes."""
+    g = cases.hub_graph() if kind == "dense_span" else cases.sparse_hub_graph()
+    print(f"[gpu] hub {kind} n={g.n} m={g.m}", flush=True)
     ref = oracle.ec_unweighted(g)
     assert_bits_equal(run(g, cuda, ec=True), ref, "hub EC")
     assert_bits_equal(run(g, cuda, ec=False), ref, "hub VC")

@@ -101,6 +101,9 @@ __device__ __forceinline__ bool exact_f64(const EngineArgs& a) {
 #ifndef JC_F64FAST
 #define JC_F64FAST 1
 #endif
+#ifndef JC_MIRRORCOUNT
+#define JC_MIRRORCOUNT 1
+#endif
 constexpr int kBucket = 8;
 constexpr int kBucketKeys = 7;
 constexpr int32_t kSpill = -2;
@@ -221,20 +224,24 @@ __device__ __forceinline__ void build_table(const Table<W>& tab, const int32_t* 
 // the mirror position q. Called by one lane.
 template <int MODE, bool W>
 __device__ __forceinline__ void emit(const EngineArgs& a, int32_t u, int du, int32_t x, int dx,
-                                     int64_t k, int64_t q, uint32_t cnt, DD sum) {
+                                     int64_t k, int64_t q, uint32_t cnt, DD sum, double su, double sx) {
+  // su, sx: the row sums of u and x (S64, or work32 for kModeIsF32), loaded
+  // by the caller ahead of time
+  (void)u;
+  (void)x;
   if constexpr (MODE == kModeEcU) {
     const float fI = (float)cnt;
     const float r = fI / ((float)(du + dx) - fI);
     a.out32[k] = r;
     if (q >= 0) a.out32[q] = r;
   } else if constexpr (MODE == kModeEcW) {
-    const double ws = a.S64[u] + a.S64[x];
+    const double ws = su + sx;
     const float r = (float)(sum.hi / (ws - sum.hi));
     a.out32[k] = r;
     if (q >= 0) a.out32[q] = r;
   } else if constexpr (MODE == kModeIsF32) {
     const float add = W ? (float)sum.hi : (float)cnt;
-    const float ws = a.work32[u] + a.work32[x];
+    const float ws = (float)su + (float)sx;
     a.out32[k] = (a.accumulate ? a.out32[k] : 0.0f) + add;
     a.ws32[k] = ws;
     if (q >= 0) {
@@ -242,7 +249,7 @@ __device__ __forceinline__ void emit(const EngineArgs& a, int32_t u, int du, int
       a.ws32[q] = ws;
     }
   } else {
-    const double ws = a.S64[u] + a.S64[x];
+    const double ws = su + sx;
     a.wi64[k] = sum.hi;
     a.ws64[k] = ws;
     if (q >= 0) {
@@ -250,6 +257,17 @@ __device__ __forceinline__ void emit(const EngineArgs& a, int32_t u, int du, int
       a.ws64[q] = ws;
     }
   }
+}
+
+// Row sum of vertex r used by the epilogue of MODE (0 when none is needed).
+template <int MODE>
+__device__ __forceinline__ double row_sum_of(const EngineArgs& a, int32_t r) {
+  if constexpr (MODE == kModeEcW || MODE == kModeIsF64)
+    return a.S64[r];
+  else if constexpr (MODE == kModeIsF32)
+    return (double)a.work32[r];
+  else
+    return 0.0;
 }
 
 // ---------------------------------------------------------------------------
@@ -371,6 +389,7 @@ struct Chunk {
   int32_t lo, hi;   // id range of the chunk's columns
   int c, nch;       // chunk index / count
   int64_t kbase;    // position of task entry j is kbase + j
+  double su;        // row sum of u used by the epilogue
   // multi-chunk scratch, indexed by item: resume position and partial result
   int* cursor;
   uint32_t* pcnt;
@@ -526,9 +545,15 @@ __device__ __forceinline__ void unit(const EngineArgs& a, const Chunk<W>& cx, bo
   int s = ox;
   if constexpr (MULTI) s = (valid && cx.c > 0) ? cx.cursor[item] : ox;
   int e = valid ? end : s;
-  // rows streamed in one group pass count the entries below u on the way;
-  // longer rows locate u with one group search
-  constexpr bool kCountMirror = !MULTI && G * U <= 64;
+  // the epilogue's row sum of x, loaded while the row streams
+  double sx = 0.0;
+  if constexpr (MODE != kModeEcU) {
+    if (valid && gl == 0) sx = row_sum_of<MODE>(a, x);
+  }
+  // single-chunk units count the entries below u while streaming (the mirror
+  // position); multi-chunk units locate u with one group search at the end
+  constexpr bool kCountMirror = !MULTI && (JC_MIRRORCOUNT || G * U <= 64);
+  constexpr bool kOnePass = G * U <= 64;
   int cnt = 0, below = 0, found = 0;
   DD acc{0.0, 0.0};
   int stop_at = e;
@@ -538,12 +563,16 @@ __device__ __forceinline__ void unit(const EngineArgs& a, const Chunk<W>& cx, bo
     stream<W, G, U, FAST, kCountMirror, false, BM>(a, cx, s, e, cnt, acc, below, found);
   (void)gm;
   (void)gl;
-  if constexpr (kCountMirror) {
+  if constexpr (kCountMirror && kOnePass) {
     // one pass of at most U entries per lane: the three counts fit a byte each
     const int packed = grp_sum_i<G>(cnt | (below << 8) | (found << 16));
     cnt = packed & 0xFF;
     below = (packed >> 8) & 0xFF;
     found = packed >> 16;
+  } else if constexpr (kCountMirror) {
+    cnt = grp_sum_i<G>(cnt);
+    below = grp_sum_i<G>(below);
+    found = (ballot(found != 0) & gm) != 0 ? 1 : 0;
   } else {
     cnt = grp_sum_i<G>(cnt);
   }
@@ -562,7 +591,7 @@ __device__ __forceinline__ void unit(const EngineArgs& a, const Chunk<W>& cx, bo
       if (valid) pos = grp_lower_bound<G>(a.ind, ox, end, cx.u);
       if (valid && pos < end && a.ind[pos] == cx.u) q = pos;
     }
-    if (valid && gl == 0) emit<MODE, W>(a, cx.u, cx.du, x, dx, cx.kbase + j, q, (uint32_t)cnt, acc);
+    if (valid && gl == 0) emit<MODE, W>(a, cx.u, cx.du, x, dx, cx.kbase + j, q, (uint32_t)cnt, acc, cx.su, sx);
   } else {
     if (!last) {
       if (valid && gl == 0) {
@@ -582,7 +611,7 @@ __device__ __forceinline__ void unit(const EngineArgs& a, const Chunk<W>& cx, bo
         else
           tot += cx.pcnt[item];
         const bool f = q < end && a.ind[q] == cx.u;
-        emit<MODE, W>(a, cx.u, cx.du, x, dx, cx.kbase + j, f ? (int64_t)q : -1, tot, acc);
+        emit<MODE, W>(a, cx.u, cx.du, x, dx, cx.kbase + j, f ? (int64_t)q : -1, tot, acc, cx.su, sx);
       }
     }
   }
@@ -711,6 +740,7 @@ __global__ __launch_bounds__(BLOCK) void k_owner_small(EngineArgs a) {
     lds_fence();
     cx.u = u;
     cx.du = du;
+    cx.su = row_sum_of<MODE>(a, u);
     cx.lo = a.ind[ou];
     cx.hi = a.ind[ou + du - 1];
     cx.c = 0;
@@ -913,6 +943,7 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
       Chunk<W> cx;
       cx.u = u;
       cx.du = du;
+      cx.su = row_sum_of<MODE>(a, u);
       cx.lo = lo0;
       cx.hi = hi0;
       cx.c = 0;
@@ -954,6 +985,7 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
       __syncthreads();
       cx.u = u;
       cx.du = du;
+      cx.su = row_sum_of<MODE>(a, u);
       cx.lo = a.ind[cs];
       cx.hi = a.ind[ce - 1];
       cx.c = c;
@@ -1064,7 +1096,7 @@ __global__ __launch_bounds__(BLOCK) void k_fixup(EngineArgs a) {
       if (lane_id() == 0) {
         const int du = a.off[u + 1] - a.off[u];
         const int dv = a.off[v + 1] - a.off[v];
-        emit<MODE, W>(a, u, du, v, dv, kk, -1, cnt, acc);
+        emit<MODE, W>(a, u, du, v, dv, kk, -1, cnt, acc, row_sum_of<MODE>(a, u), row_sum_of<MODE>(a, v));
       }
     }
   }

@@ -40,8 +40,4 @@ JC_HD uint32_t hash24(int32_t y) { return mul_u24_hash((uint32_t)y) & 0xFFFFFFu;
 
 JC_HD uint32_t hash_bucket(int32_t y, int lb) { return hash24(y) >> (24 - lb); }
 
-// Fingerprint bit of a key inside its bucket's 32-bit mask: the five hash
-// bits below the bucket bits (lb <= 19).
-JC_HD uint32_t hash_fp_bit(uint32_t h24, int lb) { return 1u << ((h24 >> (19 - lb)) & 31u); }
-
 }  // namespace jc

@@ -192,6 +192,40 @@ def run_served(args):
     return out
 
 
+def run_ingest(args):
+    """Graph ingest of the RMAT-22 ef16 draws (6.7e7 directed edges):
+    jc_csr_from_edges on the device (inputs resident) vs the host C++/OpenMP
+    builder, same CSR."""
+    import ctypes
+    import numpy as np
+    import torch
+    from jaccard_amd import api, graphs
+    from jaccard_amd._native import load_graph_lib
+    lib = load_graph_lib()
+    scale, ef = 22, 16
+    e = ef << scale
+    s = np.empty(e, np.int32)
+    d = np.empty(e, np.int32)
+    lib.jcg_rmat_edges(scale, 0, float(ef), graphs.RMAT_A, graphs.RMAT_B, graphs.RMAT_C, 1, s.ctypes.data,
+                       d.ctypes.data, e)
+    dev = torch.device("cuda:0")
+    ds, dd = torch.from_numpy(s).to(dev), torch.from_numpy(d).to(dev)
+    api.csr_from_edges(1 << scale, ds, dd)  # warm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g = api.csr_from_edges(1 << scale, ds, dd)
+    torch.cuda.synchronize()
+    t_dev = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    h = graphs.from_edges(1 << scale, s, d)
+    t_host = time.perf_counter() - t0
+    same = bool(np.array_equal(g.indices.cpu().numpy(), h.indices) and np.array_equal(g.offsets.cpu().numpy(), h.offsets))
+    print(f"[bench] ingest: device {t_dev * 1e3:.1f} ms, host {t_host * 1e3:.1f} ms, same {same}", file=sys.stderr,
+          flush=True)
+    return {"edges_in": e, "stored_edges": int(g.m), "device_ms": t_dev * 1e3, "host_ms": t_host * 1e3,
+            "host_threads": os.cpu_count(), "identical_csr": same}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -206,6 +240,7 @@ def main():
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--served", action="store_true", help="also run the serving workload")
+    ap.add_argument("--ingest", action="store_true", help="also time device graph ingest of the RMAT-22 draws")
     ap.add_argument("--serve-requests", type=int, default=512)
     ap.add_argument("--serve-qps", type=float, default=0.0, help="Poisson rate (0: 0.7 x plain capacity)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
@@ -285,6 +320,9 @@ def main():
 
     if args.served:
         extras["served"] = run_served(args)
+
+    if args.ingest:
+        extras["ingest"] = run_ingest(args)
 
     if args.calibrate_copy:
         src = torch.ones(args.calibrate_copy // 4, dtype=torch.float32, device=dev)

@@ -185,6 +185,22 @@ int jc_ec_scan(int32_t m, int32_t n, const int32_t *offsets, int32_t *dest_ind, 
 int jc_ec_unweighted(int32_t m, int32_t n, const int32_t *offsets, const int32_t *indices,
                      const int32_t *dest_ind, float *weight_j, jc_stream_t stream);
 
+/* ---- Graph ingest -------------------------------------------------------- */
+
+/* The driver's graph preparation (main.cpp:98-120; readMtxToCSR.cpp:116-208,
+ * there with std::set on the host), on the device: from num_edges directed
+ * edges (src, dst, optional edge_w; device arrays) builds a CSR with columns
+ * sorted, self loops dropped and repeated (src, dst) merged, adding the
+ * reverse of every edge when symmetrize != 0. A merged pair keeps the weight
+ * of its first occurrence, originals before reverses. offsets: n + 1 entries;
+ * indices and weights_out (NULL: no weights): capacity entries, where
+ * num_edges (2 * num_edges with symmetrize) always suffices; *m_out receives
+ * the stored edge count. Ids outside [0, n) give JC_ERR_INVALID_ARG, a
+ * capacity below the count JC_ERR_WORKSPACE. Synchronous. */
+int jc_csr_from_edges(int32_t n, int64_t num_edges, const int32_t *src, const int32_t *dst,
+                      const float *edge_w, int symmetrize, int32_t *offsets, int32_t *indices,
+                      float *weights_out, int64_t capacity, int64_t *m_out, jc_stream_t stream);
+
 /* ---- Utilities (no reference counterpart) ------------------------------ */
 
 /* Synchronous CSR check: JC_OK or JC_ERR_INVALID_CSR. *what (optional)

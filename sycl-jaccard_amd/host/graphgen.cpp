@@ -168,6 +168,25 @@ jcg_graph* jcg_rmat(int scale, int32_t n_override, double edge_factor, double a,
   return g;
 }
 
+// The raw RMAT draws of jcg_rmat (same seed and parameters give the same
+// edges, before symmetrisation and dedupe): src/dst caller-allocated with
+// round(edge_factor * n) entries. Returns that count, or -1.
+int64_t jcg_rmat_edges(int scale, int32_t n_override, double edge_factor, double a, double b, double c,
+                       uint64_t seed, int32_t* src, int32_t* dst, int64_t capacity) {
+  if (scale < 1 || scale > 30) return -1;
+  int32_t n = n_override > 0 ? n_override : (int32_t)(1u << scale);
+  if ((int64_t)n > (1ll << scale)) return -1;
+  int64_t e = (int64_t)std::llround(edge_factor * (double)n);
+  if (e > capacity) return -1;
+  const double two32 = 4294967296.0;
+  uint32_t ta = (uint32_t)std::min(two32 - 1, std::floor(a * two32));
+  uint32_t tab = (uint32_t)std::min(two32 - 1, std::floor((a + b) * two32));
+  uint32_t tabc = (uint32_t)std::min(two32 - 1, std::floor((a + b + c) * two32));
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < e; ++i) rmat_edge(seed, scale, n, (uint64_t)i, ta, tab, tabc, &src[i], &dst[i]);
+  return e;
+}
+
 // Chung-Lu graph with a power-law expected-degree sequence
 //   w_i = w_max * (1 + i/i0)^(-1/(gamma-1)),  i in [0,n)
 // where i0 is solved so that mean(w) = mean_deg. Endpoints of each sampled edge

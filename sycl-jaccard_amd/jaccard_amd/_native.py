@@ -65,6 +65,8 @@ def load_graph_lib() -> ctypes.CDLL:
             lib.jcg_pair_weight.argtypes = [u64, i32, i32]
             lib.jcg_rmat_batch.restype = vp
             lib.jcg_rmat_batch.argtypes = [i32, vp, vp, vp, ctypes.c_int, u64, f64, f64, f64, vp, vp]
+            lib.jcg_rmat_edges.restype = i64
+            lib.jcg_rmat_edges.argtypes = [ctypes.c_int, i32, f64, f64, f64, f64, u64, vp, vp, i64]
             lib.jcio_load.restype = vp
             lib.jcio_load.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
             for name, rt in (("jcio_n", i64), ("jcio_m", i64), ("jcio_flags", u64)):
@@ -109,6 +111,7 @@ def _declare_jaccard(lib: ctypes.CDLL) -> None:
         "jc_ec_unweighted": [i32, i32, vp, vp, vp, vp, vp],
         "jc_validate_csr": [vp, ctypes.POINTER(I), vp],
         "jc_csr_is_symmetric": [vp, ctypes.POINTER(I), vp],
+        "jc_csr_from_edges": [i32, i64, vp, vp, vp, I, vp, vp, vp, i64, ctypes.POINTER(i64), vp],
         "jc_profile_enable": [I],
         "jc_profile_reset": [],
         "jc_profile_query": [ctypes.c_char_p, ctypes.POINTER(I), ctypes.POINTER(ctypes.c_double)],

@@ -235,6 +235,31 @@ def validate_csr(graph: DeviceCSR, stream=None) -> int:
     return int(what.value)
 
 
+def csr_from_edges(n: int, src, dst, edge_weights=None, symmetrize: bool = True, weighted_out: bool = None,
+                   stream=None) -> DeviceCSR:
+    """Device graph ingest (jc_csr_from_edges): src/dst/edge_weights are device
+    int32/float32 tensors of one directed edge each; returns a sorted,
+    duplicate-free CSR without self loops (reverse edges added when
+    symmetrize). Weights are produced when edge_weights is given (or
+    weighted_out)."""
+    torch = _torch()
+    lib = load_jaccard_lib()
+    e = int(src.numel())
+    dev = src.device
+    cap = max(e * (2 if symmetrize else 1), 1)
+    off = torch.empty(n + 1, dtype=torch.int32, device=dev)
+    ind = torch.empty(cap, dtype=torch.int32, device=dev)
+    want_w = edge_weights is not None if weighted_out is None else weighted_out
+    w = torch.empty(cap, dtype=torch.float32, device=dev) if want_w else None
+    m = ctypes.c_int64(0)
+    _check(lib.jc_csr_from_edges(n, e, src.data_ptr(), dst.data_ptr(),
+                                 None if edge_weights is None else edge_weights.data_ptr(), 1 if symmetrize else 0,
+                                 off.data_ptr(), ind.data_ptr(), None if w is None else w.data_ptr(), cap,
+                                 ctypes.byref(m), _stream(stream)), "jc_csr_from_edges")
+    mm = int(m.value)
+    return DeviceCSR(n, off, ind[:mm], None if w is None else w[:mm])
+
+
 def is_symmetric(graph: DeviceCSR, stream=None) -> bool:
     lib = load_jaccard_lib()
     sym = ctypes.c_int(0)

@@ -81,3 +81,21 @@ def test_validate_catches_bad_csr():
     bad.indices[0] = g.n
     with pytest.raises(ValueError):
         bad.validate()
+
+
+def test_rmat_raw_edges_are_the_generator_draws():
+    import ctypes
+    from jaccard_amd._native import load_graph_lib
+    lib = load_graph_lib()
+    scale, ef = 10, 8
+    e = ef << scale
+    s = np.empty(e, np.int32)
+    d = np.empty(e, np.int32)
+    assert lib.jcg_rmat_edges(scale, 0, float(ef), graphs.RMAT_A, graphs.RMAT_B, graphs.RMAT_C, 5,
+                              s.ctypes.data, d.ctypes.data, e) == e
+    assert s.min() >= 0 and s.max() < (1 << scale) and d.max() < (1 << scale)
+    want = graphs.rmat(scale, ef, seed=5)
+    got = graphs.from_edges(1 << scale, s, d)
+    assert np.array_equal(got.offsets, want.offsets) and np.array_equal(got.indices, want.indices)
+    assert lib.jcg_rmat_edges(scale, 0, float(ef), graphs.RMAT_A, graphs.RMAT_B, graphs.RMAT_C, 5,
+                              s.ctypes.data, d.ctypes.data, e - 1) == -1

@@ -474,55 +474,80 @@ __device__ __forceinline__ void probe_u(const Chunk<W>& cx, const int32_t (&y)[U
 // (entries of that pass beyond it are probed too, and miss). Returns the
 // stop position, or e.
 template <bool W, int G, int U, bool FAST, bool COUNT, bool STOP, bool BM>
+__device__ __forceinline__ void probe_pass(const Chunk<W>& cx, int32_t (&y)[U], int& cnt, DD& acc, int& below,
+                                           int& found) {
+  if constexpr (COUNT) {
+#pragma unroll
+    for (int t = 0; t < U; ++t) {
+      below += y[t] < cx.u ? 1 : 0;
+      found |= y[t] == cx.u ? 1 : 0;
+    }
+  }
+  if constexpr (BM)
+    probe_bm<W, U, FAST>(cx, y, cnt, acc);
+  else
+    probe_u<W, U, FAST>(cx, y, cnt, acc);
+}
+
+// STOP: position of the first entry of the pass at p0 above cx.hi, or -1 when
+// the pass does not cross the owner chunk's id bound (rows are sorted: it does
+// iff its last entry does).
+template <int G, int U>
+__device__ __forceinline__ int pass_stop(const int32_t (&y)[U], int32_t hi, int p0, int e) {
+  const uint64_t gm = group_mask<G>();
+  if (!(ballot(y[U - 1] > hi) & gm)) return -1;
+  const int gb = lane_id() & ~(G - 1);
+  int first = e;
+#pragma unroll
+  for (int t = U - 1; t >= 0; --t) {
+    const uint64_t bm = ballot(y[t] > hi) & gm;
+    if (bm) first = p0 + t * G + (__ffsll((unsigned long long)bm) - 1 - gb);
+  }
+  return min(first, e);
+}
+
+// Streams entries [s, e) of a short row (bounds uniform in the group of G
+// lanes), U loads per lane per pass. While a further pass exists its loads
+// are issued before the current pass is probed (global latency overlaps the
+// LDS probes); the loop issues them on every iteration, so the wait for the
+// current entries never waits for the ones just issued. The last pass
+// (partial, clamped lanes probe a key that is never stored) runs after the
+// loop. COUNT: also count the entries below u and whether u occurs (mirror
+// position). STOP: the owner's chunk ends at id cx.hi; stop after the pass
+// that crosses it and return the position of the first entry above cx.hi
+// (entries of that pass beyond it are probed too, and miss). Returns the
+// stop position, or e.
+template <bool W, int G, int U, bool FAST, bool COUNT, bool STOP, bool BM, bool ONEPASS = false>
 __device__ __forceinline__ int stream(const EngineArgs& a, const Chunk<W>& cx, int s, int e, int& cnt, DD& acc,
                                       int& below, int& found) {
   const int gl = lane_id() & (G - 1);
-  int32_t y[U];
   int p0 = s;
-  if (p0 < e) {
+  if (p0 >= e) return e;
+  int32_t y[U];
 #pragma unroll
-    for (int t = 0; t < U; ++t) y[t] = a.ind[min(p0 + t * G + gl, e - 1)];
-  }
-  while (p0 < e) {
+  for (int t = 0; t < U; ++t) y[t] = a.ind[min(p0 + t * G + gl, e - 1)];
+  // ONEPASS: the caller knows every row fits one pass (e - s <= G * U)
+  while (!ONEPASS && p0 + G * U < e) {  // the pass at p0 is full and another follows
     const int pn = p0 + G * U;
     int32_t yn[U];
-    if (pn < e) {
 #pragma unroll
-      for (int t = 0; t < U; ++t) yn[t] = a.ind[min(pn + t * G + gl, e - 1)];
-    }
-    if (pn > e) {  // partial last pass: clamped lanes probe a key that is never stored
-#pragma unroll
-      for (int t = 0; t < U; ++t)
-        if (p0 + t * G + gl >= e) y[t] = INT32_MAX;
-    }
-    if constexpr (COUNT) {
-#pragma unroll
-      for (int t = 0; t < U; ++t) {
-        below += y[t] < cx.u ? 1 : 0;
-        found |= y[t] == cx.u ? 1 : 0;
-      }
-    }
-    if constexpr (BM)
-      probe_bm<W, U, FAST>(cx, y, cnt, acc);
-    else
-      probe_u<W, U, FAST>(cx, y, cnt, acc);
+    for (int t = 0; t < U; ++t) yn[t] = a.ind[min(pn + t * G + gl, e - 1)];
+    probe_pass<W, G, U, FAST, COUNT, STOP, BM>(cx, y, cnt, acc, below, found);
     if constexpr (STOP) {
-      const uint64_t gm = group_mask<G>();
-      // rows are sorted: the pass crosses the bound iff its last entry does
-      if (ballot(y[U - 1] > cx.hi) & gm) {
-        const int gb = lane_id() & ~(G - 1);
-        int first = e;
-#pragma unroll
-        for (int t = U - 1; t >= 0; --t) {
-          const uint64_t bm = ballot(y[t] > cx.hi) & gm;
-          if (bm) first = p0 + t * G + (__ffsll((unsigned long long)bm) - 1 - gb);
-        }
-        return min(first, e);
-      }
+      const int st = pass_stop<G, U>(y, cx.hi, p0, e);
+      if (st >= 0) return st;
     }
     p0 = pn;
 #pragma unroll
     for (int t = 0; t < U; ++t) y[t] = yn[t];
+  }
+#pragma unroll
+  for (int t = 0; t < U; ++t)
+    if (p0 + t * G + gl >= e) y[t] = INT32_MAX;
+  probe_pass<W, G, U, FAST, COUNT, STOP, BM>(cx, y, cnt, acc, below, found);
+  if constexpr (STOP) {
+    const int st = pass_stop<G, U>(y, cx.hi, p0, e);
+    if (st >= 0) return st;
   }
   return e;
 }
@@ -535,7 +560,7 @@ __device__ __forceinline__ int stream(const EngineArgs& a, const Chunk<W>& cx, i
 //     chunk's id bound, store the partial and the resume position; on the
 //     last chunk stream to the end, add the stored partial, locate u by one
 //     group search and emit.
-template <int MODE, bool W, int G, int U, bool MULTI, bool FAST, bool BM>
+template <int MODE, bool W, int G, int U, bool MULTI, bool FAST, bool BM, bool SMALL = false>
 __device__ __forceinline__ void unit(const EngineArgs& a, const Chunk<W>& cx, bool valid, int32_t x,
                                      int ox, int dx, int j, int item) {
   const int gl = lane_id() & (G - 1);
@@ -560,7 +585,8 @@ __device__ __forceinline__ void unit(const EngineArgs& a, const Chunk<W>& cx, bo
   if (MULTI && !last)
     stop_at = stream<W, G, U, FAST, kCountMirror, true, BM>(a, cx, s, e, cnt, acc, below, found);
   else
-    stream<W, G, U, FAST, kCountMirror, false, BM>(a, cx, s, e, cnt, acc, below, found);
+    stream<W, G, U, FAST, kCountMirror, false, BM, SMALL || (!MULTI && kOnePass)>(a, cx, s, e, cnt, acc, below,
+                                                                                 found);
   (void)gm;
   (void)gl;
   if constexpr (kCountMirror && kOnePass) {
@@ -635,7 +661,7 @@ __host__ __device__ __forceinline__ constexpr int class_g(int c) {
   return c == 0 ? 8 : c <= 3 ? 16 : c == 4 ? 32 : 64;
 }
 
-template <int MODE, bool W, bool MULTI, int C, bool BM = false>
+template <int MODE, bool W, bool MULTI, int C, bool BM = false, bool SMALL = false>
 __device__ __forceinline__ void unit_class(const EngineArgs& a, const Chunk<W>& cx, bool valid, int32_t x,
                                            int ox, int dx, int j, int item, bool fast) {
   constexpr int G = class_g(C);
@@ -643,16 +669,18 @@ __device__ __forceinline__ void unit_class(const EngineArgs& a, const Chunk<W>& 
   // classes 0-3 stream a row (at most 64 entries) in one pass of G * U; the
   // single-pass mirror count and its byte-packed reduction rely on it
   static_assert(C > 3 ? G * U > 64 : G * U >= (C == 0 ? 8 : C == 1 ? 16 : C == 2 ? 32 : 64), "class shape");
+  // rows of the one-wave-per-row kernel (<= kSmallMax entries) fit one pass
+  static_assert(!SMALL || C > 4 || G * U >= (C == 4 ? kSmallMax : 0), "small rows fit one pass");
 #if JC_F64FAST
   if constexpr (W) {
     if (fast) {
-      unit<MODE, W, G, U, MULTI, true, BM>(a, cx, valid, x, ox, dx, j, item);
+      unit<MODE, W, G, U, MULTI, true, BM, SMALL>(a, cx, valid, x, ox, dx, j, item);
       return;
     }
   }
 #endif
   (void)fast;
-  unit<MODE, W, G, U, MULTI, false, BM>(a, cx, valid, x, ox, dx, j, item);
+  unit<MODE, W, G, U, MULTI, false, BM, SMALL>(a, cx, valid, x, ox, dx, j, item);
 }
 
 // Items of one class held in lanes (x, ox, dx, j per lane; mask = owned lanes
@@ -678,7 +706,9 @@ __device__ __forceinline__ void run_class_regs(const EngineArgs& a, const Chunk<
     const int oxx = __shfl(ox, s);
     const int dxx = __shfl(dx, s);
     const int jj = __shfl(j, s);
-    unit_class<MODE, W, false, C>(a, cx, src >= 0, xx, oxx, dxx, jj, 0, fast);
+    // rows here have at most kSmallMax entries: one pass of every class of
+    // such rows covers them (class 4: G * U = 128)
+    unit_class<MODE, W, false, C, false, true>(a, cx, src >= 0, xx, oxx, dxx, jj, 0, fast);
   }
 }
 

@@ -101,6 +101,9 @@ __device__ __forceinline__ bool exact_f64(const EngineArgs& a) {
 #ifndef JC_F64FAST
 #define JC_F64FAST 1
 #endif
+#ifndef JC_VALCOND
+#define JC_VALCOND 1
+#endif
 #ifndef JC_MIRRORCOUNT
 #define JC_MIRRORCOUNT 1
 #endif
@@ -171,8 +174,12 @@ struct Table {
       slot = e3 ? 3 : slot;
       slot = e2 ? 2 : slot;
       slot = e1 ? 1 : slot;
+#if JC_VALCOND
+      if (h.hit) h.v = val[b * kBucket + slot];  // only hit lanes read the weight
+#else
       const float v = val[b * kBucket + slot];
       h.v = h.hit ? v : 0.0f;
+#endif
     }
 #endif
     return h;

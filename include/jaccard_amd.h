@@ -103,7 +103,11 @@ typedef struct jc_tuning {
                                   wave per row, longer rows one workgroup per task
                                   (1..128, default 128) */
   int32_t small_blocks_per_cu; /* grid of the one-wave-per-row kernel (default 16) */
-  int32_t reserved[6];
+  int32_t wide_index;          /* nonzero: stream rows with 64-bit addresses even when
+                                  the index array is small enough for 32-bit buffer
+                                  offsets (arrays of 2^30 entries or more always use
+                                  them); for testing both load paths */
+  int32_t reserved[5];
 } jc_tuning;
 
 /* jc_jaccard_ex with launch tuning (NULL = defaults). */

@@ -91,22 +91,27 @@ __device__ __forceinline__ bool exact_f64(const EngineArgs& a) {
 // ---------------------------------------------------------------------------
 // Bucketized LDS hash table: 2^lb buckets of kBucket int32 slots. Slots 0..6
 // hold keys (-1 = empty); slot 7 is the spill marker, set when a key had to
-// move on to the next bucket. Weighted tables keep the fp32 weight of slot i
-// at val[i], so a probe reads keys and weights of one bucket with four
-// ds_read_b128 and selects with the same compares. Bucket = top bits of a
-// multiplicative hash.
-#ifndef JC_WVAL_SAMEPASS
-#define JC_WVAL_SAMEPASS 0
-#endif
+// move on to the next bucket. Weighted tables keep the fp32 weight of key
+// slot i of bucket b at val[8b + i]; val[7] (bucket 0's unused slot 7) holds
+// 0.0f and is what a missing lane reads, so every lane issues the same one
+// weight read without a branch and the missing lanes share one LDS word
+// (a broadcast, no bank conflicts). Bucket = top bits of a multiplicative
+// hash.
 #ifndef JC_F64FAST
 #define JC_F64FAST 1
-#endif
-#ifndef JC_VALCOND
-#define JC_VALCOND 1
 #endif
 #ifndef JC_MIRRORCOUNT
 #define JC_MIRRORCOUNT 1
 #endif
+// c ? a : b per lane as one v_cndmask_b32 that the compiler cannot turn into
+// a branch (it would otherwise sink the computation of a into one).
+__device__ __forceinline__ uint32_t lane_select(bool c, uint32_t a, uint32_t b) {
+  const uint64_t m = __builtin_amdgcn_ballot_w64(c);
+  uint32_t r;
+  asm("v_cndmask_b32 %0, %1, %2, %3" : "=v"(r) : "v"(b), "v"(a), "s"(m));
+  return r;
+}
+
 constexpr int kBucket = 8;
 constexpr int kBucketKeys = 7;
 constexpr int32_t kSpill = -2;
@@ -126,6 +131,10 @@ struct Table {
   __device__ __forceinline__ uint32_t bucket(int32_t y) const { return hash_bucket(y, lb); }
   __device__ __forceinline__ uint32_t next(uint32_t b) const { return (b + 1) & bmask; }
   __device__ __forceinline__ void clear(int i) const { key[i] = -1; }
+  // the miss word; written once per table build, before any probe
+  __device__ __forceinline__ void set_zero() const {
+    if constexpr (W) val[kBucketKeys] = 0.0f;
+  }
   __device__ __forceinline__ void insert(int32_t y, float v) const {
     uint32_t b = bucket(y);
     for (;;) {
@@ -142,6 +151,10 @@ struct Table {
       b = next(b);
     }
   }
+  // Straight-line probe of bucket b: two ds_read_b128 for the keys, seven
+  // compares; weighted tables add one ds_read_b32 at the hit slot (slot
+  // index from three masks: bit 0 = slots 1, 3, 5; bit 1 = 2, 3, 6; bit 2 =
+  // 4, 5, 6) or at the miss word.
   __device__ __forceinline__ Hit look(uint32_t b, int32_t y) const {
     const int4* kp = reinterpret_cast<const int4*>(key + b * kBucket);
     const int4 k0 = kp[0];
@@ -152,36 +165,11 @@ struct Table {
     h.hit = e0 | e1 | e2 | e3 | e4 | e5 | e6;
     h.spill = k1.w == kSpill;
     h.v = 0.0f;
-#if JC_WVAL_SAMEPASS
     if constexpr (W) {
-      const float4* vp = reinterpret_cast<const float4*>(val + b * kBucket);
-      const float4 v0 = vp[0];
-      const float4 v1 = vp[1];
-      float v = e6 ? v1.z : 0.0f;
-      v = e5 ? v1.y : v;
-      v = e4 ? v1.x : v;
-      v = e3 ? v0.w : v;
-      v = e2 ? v0.z : v;
-      v = e1 ? v0.y : v;
-      v = e0 ? v0.x : v;
-      h.v = v;
+      const bool m1 = e1 | e3 | e5, m2 = e2 | e3 | e6, m4 = e4 | e5 | e6;
+      const uint32_t slot = (m1 ? 1u : 0u) + (m2 ? 2u : 0u) + (m4 ? 4u : 0u);
+      h.v = val[lane_select(h.hit, b * kBucket + slot, (uint32_t)kBucketKeys)];
     }
-#else
-    if constexpr (W) {
-      int slot = e6 ? 6 : 0;
-      slot = e5 ? 5 : slot;
-      slot = e4 ? 4 : slot;
-      slot = e3 ? 3 : slot;
-      slot = e2 ? 2 : slot;
-      slot = e1 ? 1 : slot;
-#if JC_VALCOND
-      if (h.hit) h.v = val[b * kBucket + slot];  // only hit lanes read the weight
-#else
-      const float v = val[b * kBucket + slot];
-      h.v = h.hit ? v : 0.0f;
-#endif
-    }
-#endif
     return h;
   }
 };
@@ -401,24 +389,35 @@ struct Chunk {
   int* cursor;
   uint32_t* pcnt;
   DD* pacc;
-  // bitmap table (rows whose id span fits the LDS): bit (y - lo) of bits is
-  // set for every key y; weighted: rank[w] = keys in the words before w and
-  // bval[r] = weight of the r-th key in id order
+  // bitmap table (rows whose id span fits the LDS): bit (y - lo) is set for
+  // every key y. Unweighted: word i at bits[i]. Weighted: word i at bits[2i]
+  // and the number of keys in the words before it at bits[2i + 1] (one
+  // 8-byte read per probe); bval[r] = weight of the r-th key in id order and
+  // bval[bzero] = 0.0f, the word a missing lane reads.
   const uint32_t* bits;
-  const uint32_t* rank;
   const float* bval;
   uint32_t span;    // hi - lo + 1
+  uint32_t bzero;
 };
 
-// Probes U elements per lane against a bitmap table: one LDS word per probe;
-// hits of weighted tables read the key's rank and weight.
+// Probes U elements per lane against a bitmap table: one LDS read per probe
+// (weighted: the word and its rank together), then weighted tables read the
+// weight at rank + keys below the bit in the word, or the zero word.
 template <bool W, int U, bool FAST>
 __device__ __forceinline__ void probe_bm(const Chunk<W>& cx, const int32_t (&y)[U], int& cnt, DD& acc) {
-  uint32_t off[U], wd[U];
+  uint32_t off[U], wd[U], rk[U];
 #pragma unroll
   for (int t = 0; t < U; ++t) {
     off[t] = (uint32_t)(y[t] - cx.lo);  // below lo wraps to a large value
-    wd[t] = cx.bits[off[t] < cx.span ? off[t] >> 5 : 0u];
+    const uint32_t wi = off[t] < cx.span ? off[t] >> 5 : 0u;
+    if constexpr (W) {
+      const uint2 p = reinterpret_cast<const uint2*>(cx.bits)[wi];
+      wd[t] = p.x;
+      rk[t] = p.y;
+    } else {
+      wd[t] = cx.bits[wi];
+      rk[t] = 0u;
+    }
   }
 #pragma unroll
   for (int t = 0; t < U; ++t) {
@@ -426,8 +425,8 @@ __device__ __forceinline__ void probe_bm(const Chunk<W>& cx, const int32_t (&y)[
     const bool hit = off[t] < cx.span && ((wd[t] >> sh) & 1u);
     cnt += hit ? 1 : 0;
     if constexpr (W) {
-      float v = 0.0f;
-      if (hit) v = cx.bval[cx.rank[off[t] >> 5] + __popc(wd[t] & ((1u << sh) - 1u))];
+      const uint32_t r = rk[t] + __popc(wd[t] & ((1u << sh) - 1u));
+      const float v = cx.bval[hit ? r : cx.bzero];
       if constexpr (FAST)
         acc.hi += (double)v;
       else
@@ -472,14 +471,8 @@ __device__ __forceinline__ void probe_u(const Chunk<W>& cx, const int32_t (&y)[U
   }
 }
 
-// Streams entries [s, e) of a short row (bounds uniform in the group of G
-// lanes), U loads per lane per pass; the next pass's loads are issued before
-// the current pass is probed, so global latency overlaps the LDS probes.
-// COUNT: also count the entries below u and whether u occurs (mirror
-// position). STOP: the owner's chunk ends at id cx.hi; stop after the pass
-// that crosses it and return the position of the first entry above cx.hi
-// (entries of that pass beyond it are probed too, and miss). Returns the
-// stop position, or e.
+// Probes the U entries of one pass per lane (bitmap or hash table). COUNT:
+// also counts the entries below u and whether u occurs (mirror position).
 template <bool W, int G, int U, bool FAST, bool COUNT, bool STOP, bool BM>
 __device__ __forceinline__ void probe_pass(const Chunk<W>& cx, int32_t (&y)[U], int& cnt, DD& acc, int& below,
                                            int& found) {
@@ -513,32 +506,57 @@ __device__ __forceinline__ int pass_stop(const int32_t (&y)[U], int32_t hi, int 
   return min(first, e);
 }
 
+// Word 3 of a raw buffer resource on gfx9-family parts: 32-bit data format,
+// untyped dword loads.
+constexpr int kBufWord3 = 0x00020000;
+
+// Loads one pass: lane gl of the group reads entries p + t*G + gl, t < U.
+// BUF: raw buffer loads through one resource over ind with 32-bit byte
+// offsets (the launcher picks this form when 4 * m fits them). A read at or
+// past the end of ind returns 0, so a pass that runs past the row's end needs
+// no clamping (its extra values are replaced before use) and each load is one
+// offset add with the t*G*4 part in the instruction. Otherwise 64-bit
+// addresses with the index clamped to the row.
+template <bool BUF, int G, int U>
+__device__ __forceinline__ void load_pass(const EngineArgs& a, int32_t (&y)[U], int p, int e) {
+  const int gl = lane_id() & (G - 1);
+  if constexpr (BUF) {
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<int32_t*>(a.ind), (short)0, (int)(uint32_t)(a.m * 4), kBufWord3);
+    const uint32_t vo = (uint32_t)(p + gl) * 4u;
+#pragma unroll
+    for (int t = 0; t < U; ++t)
+      y[t] = (int32_t)__builtin_amdgcn_raw_buffer_load_b32(rs, (int)(vo + (uint32_t)(t * G * 4)), 0, 0);
+  } else {
+#pragma unroll
+    for (int t = 0; t < U; ++t) y[t] = a.ind[min(p + t * G + gl, e - 1)];
+  }
+}
+
 // Streams entries [s, e) of a short row (bounds uniform in the group of G
 // lanes), U loads per lane per pass. While a further pass exists its loads
 // are issued before the current pass is probed (global latency overlaps the
 // LDS probes); the loop issues them on every iteration, so the wait for the
 // current entries never waits for the ones just issued. The last pass
-// (partial, clamped lanes probe a key that is never stored) runs after the
-// loop. COUNT: also count the entries below u and whether u occurs (mirror
-// position). STOP: the owner's chunk ends at id cx.hi; stop after the pass
-// that crosses it and return the position of the first entry above cx.hi
-// (entries of that pass beyond it are probed too, and miss). Returns the
-// stop position, or e.
-template <bool W, int G, int U, bool FAST, bool COUNT, bool STOP, bool BM, bool ONEPASS = false>
-__device__ __forceinline__ int stream(const EngineArgs& a, const Chunk<W>& cx, int s, int e, int& cnt, DD& acc,
-                                      int& below, int& found) {
+// (partial: lanes past the row's end probe a key that is never stored) runs
+// after the loop. COUNT: also count the entries below u and whether u occurs
+// (mirror position). STOP: the owner's chunk ends at id cx.hi; stop after the
+// pass that crosses it and return the position of the first entry above
+// cx.hi (entries of that pass beyond it are probed too, and miss). Returns
+// the stop position, or e.
+template <bool W, int G, int U, bool FAST, bool COUNT, bool STOP, bool BM, bool ONEPASS, bool BUF>
+__device__ __forceinline__ int stream_t(const EngineArgs& a, const Chunk<W>& cx, int s, int e, int& cnt, DD& acc,
+                                        int& below, int& found) {
   const int gl = lane_id() & (G - 1);
   int p0 = s;
   if (p0 >= e) return e;
   int32_t y[U];
-#pragma unroll
-  for (int t = 0; t < U; ++t) y[t] = a.ind[min(p0 + t * G + gl, e - 1)];
+  load_pass<BUF, G, U>(a, y, p0, e);
   // ONEPASS: the caller knows every row fits one pass (e - s <= G * U)
   while (!ONEPASS && p0 + G * U < e) {  // the pass at p0 is full and another follows
     const int pn = p0 + G * U;
     int32_t yn[U];
-#pragma unroll
-    for (int t = 0; t < U; ++t) yn[t] = a.ind[min(pn + t * G + gl, e - 1)];
+    load_pass<BUF, G, U>(a, yn, pn, e);
     probe_pass<W, G, U, FAST, COUNT, STOP, BM>(cx, y, cnt, acc, below, found);
     if constexpr (STOP) {
       const int st = pass_stop<G, U>(y, cx.hi, p0, e);
@@ -557,6 +575,14 @@ __device__ __forceinline__ int stream(const EngineArgs& a, const Chunk<W>& cx, i
     if (st >= 0) return st;
   }
   return e;
+}
+
+template <bool W, int G, int U, bool FAST, bool COUNT, bool STOP, bool BM, bool ONEPASS = false>
+__device__ __forceinline__ int stream(const EngineArgs& a, const Chunk<W>& cx, int s, int e, int& cnt, DD& acc,
+                                      int& below, int& found) {
+  if (!a.wide_index)
+    return stream_t<W, G, U, FAST, COUNT, STOP, BM, ONEPASS, true>(a, cx, s, e, cnt, acc, below, found);
+  return stream_t<W, G, U, FAST, COUNT, STOP, BM, ONEPASS, false>(a, cx, s, e, cnt, acc, below, found);
 }
 
 // One unit of work: each group of G lanes handles one owned short row x
@@ -772,6 +798,7 @@ __global__ __launch_bounds__(BLOCK) void k_owner_small(EngineArgs a) {
     Chunk<W> cx;
     cx.tab = Table<W>{key_all[w], W ? val_all[W ? w : 0] : nullptr, lb, (uint32_t)((1 << lb) - 1)};
     for (int i = lane; i < (kBucket << lb); i += kWave) cx.tab.clear(i);
+    if (lane == 0) cx.tab.set_zero();
     lds_fence();
     build_table<W>(cx.tab, a.ind, a.vw, ou, du, lane, kWave);
     lds_fence();
@@ -907,7 +934,7 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
     const int32_t lo0 = a.ind[ou], hi0 = a.ind[ou + du - 1];
     const uint32_t words = ((uint32_t)(hi0 - lo0) >> 5) + 1u;
     const bool bm = JC_BITMAP &&
-                    (size_t)words * (W ? 8u : 4u) + (W ? (size_t)4 * du : 0) <= Cfg::key_bytes + Cfg::val_bytes;
+                    (size_t)words * (W ? 8u : 4u) + (W ? (size_t)4 * (du + 1) : 0) <= Cfg::key_bytes + Cfg::val_bytes;
     const int nch = bm ? 1 : (du + CH - 1) / CH;
 
     // owned entries of the task, sorted into length classes (heaviest first)
@@ -951,29 +978,31 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
     }
     const int nunits = s_ubase[kClasses];
     if (bm) {
-      // bits, then (weighted) the rank of every word and the keys' weights
+      // the words (weighted: interleaved with their ranks), then the keys'
+      // weights in id order and the zero word
+      constexpr uint32_t WS = W ? 2u : 1u;  // stride of the words
       uint32_t* bits = reinterpret_cast<uint32_t*>(smem);
-      uint32_t* brank = bits + words;
-      float* bval = reinterpret_cast<float*>(brank + (W ? words : 0u));
-      for (uint32_t i = tid; i < words; i += BLOCK) bits[i] = 0u;
+      float* bval = reinterpret_cast<float*>(bits + WS * words);
+      for (uint32_t i = tid; i < WS * words; i += BLOCK) bits[i] = 0u;
       if (tid == 0) s_next = 0;
       __syncthreads();
       for (int i = tid; i < du; i += BLOCK) {
         const int32_t y = a.ind[ou + i];
         const uint32_t o = (uint32_t)(y - lo0);
-        if (o < words * 32u) atomicOr(&bits[o >> 5], 1u << (o & 31u));  // rows are sorted: always true
+        if (o < words * 32u) atomicOr(&bits[WS * (o >> 5)], 1u << (o & 31u));  // rows are sorted: always true
         if constexpr (W) bval[i] = a.vw[y];  // the i-th key in id order has rank i
       }
+      if (W && tid == 0) bval[du] = 0.0f;
       __syncthreads();
       if constexpr (W) {
         const uint32_t per = (words + BLOCK - 1) / BLOCK;
         const uint32_t w0 = min(words, (uint32_t)tid * per), w1 = min(words, w0 + per);
         int c = 0;
-        for (uint32_t i = w0; i < w1; ++i) c += __popc(bits[i]);
+        for (uint32_t i = w0; i < w1; ++i) c += __popc(bits[2 * i]);
         int r = block_excl_scan(c, s_wsum);
         for (uint32_t i = w0; i < w1; ++i) {
-          brank[i] = (uint32_t)r;
-          r += __popc(bits[i]);
+          bits[2 * i + 1] = (uint32_t)r;
+          r += __popc(bits[2 * i]);
         }
         __syncthreads();
       }
@@ -990,9 +1019,9 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
       cx.pcnt = pcnt;
       cx.pacc = pacc;
       cx.bits = bits;
-      cx.rank = brank;
       cx.bval = bval;
       cx.span = (uint32_t)(hi0 - lo0) + 1u;
+      cx.bzero = (uint32_t)du;
       for (;;) {
         int un = 0;
         if (lane == 0) un = atomicAdd(&s_next, 1);
@@ -1016,7 +1045,10 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
       Chunk<W> cx;
       cx.tab = Table<W>{tkey, tval, lb, (uint32_t)((1 << lb) - 1)};
       for (int i = tid; i < (kBucket << lb); i += BLOCK) cx.tab.clear(i);
-      if (tid == 0) s_next = 0;
+      if (tid == 0) {
+        s_next = 0;
+        cx.tab.set_zero();
+      }
       __syncthreads();
       build_table<W>(cx.tab, a.ind, a.vw, cs, clen, tid, BLOCK);
       __syncthreads();
@@ -1275,6 +1307,9 @@ static hipError_t run_engine_t(const EngineArgs& base, void* ws, bool fixup, hip
   const int64_t edges = a.nseg > 0 ? a.seg_edges : a.m;
   if (a.small_max < 1 || a.small_max > kSmallMax) a.small_max = kSmallMax;
   if (a.small_blocks_per_cu < 1) a.small_blocks_per_cu = 16;
+  // buffer-load offsets reach 4 * (m + 256) bytes
+  if (a.m * 4 + 8192 > (int64_t)UINT32_MAX) a.wide_index = 1;
+  a.wide_index = a.wide_index ? 1 : 0;
   unsigned char* p = static_cast<unsigned char*>(ws);
   Sched* sc = reinterpret_cast<Sched*>(p);
   p += align_up(sizeof(Sched), 256);

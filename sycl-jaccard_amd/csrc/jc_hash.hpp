@@ -38,6 +38,13 @@ JC_HD uint32_t mul_u24_hash(uint32_t a) {
 // The 24-bit hash word of a key.
 JC_HD uint32_t hash24(int32_t y) { return mul_u24_hash((uint32_t)y) & 0xFFFFFFu; }
 
-JC_HD uint32_t hash_bucket(int32_t y, int lb) { return hash24(y) >> (24 - lb); }
+// Bits [24 - lb, 24) of the hash word; on the device one bit-field extract.
+JC_HD uint32_t hash_bucket(int32_t y, int lb) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_ubfe(mul_u24_hash((uint32_t)y), 24 - lb, lb);
+#else
+  return hash24(y) >> (24 - lb);
+#endif
+}
 
 }  // namespace jc

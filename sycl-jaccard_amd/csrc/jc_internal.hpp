@@ -56,6 +56,8 @@ struct EngineArgs {
   // tuning (results do not depend on these)
   int small_max;              // degree bin threshold: one wave per row up to here (1..kSmallMax)
   int small_blocks_per_cu;    // grid of the one-wave-per-row kernel
+  int wide_index;             // 1: rows are read with 64-bit addresses (set by the
+                              // launcher when m * 4 does not fit 32-bit offsets)
   // Row segments (device memory, optional): when nseg > 0 only the rows of
   // the segments are computed. segs[i] = (first row, running row count before
   // segment i) for i < nseg, and segs[nseg] = (0, total rows); seg_rows and

@@ -111,21 +111,34 @@ def _empty_f32(n, device):
     return torch.empty(max(n, 0), dtype=torch.float32, device=device)
 
 
+class JcTuning(ctypes.Structure):
+    """``jc_tuning``: launch parameters that do not change results."""
+    _fields_ = [("small_max", ctypes.c_int32), ("small_blocks_per_cu", ctypes.c_int32),
+                ("wide_index", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5)]
+
+
 def jaccard(graph: DeviceCSR, result=None, weights=None, edge_centric: bool = True, stream=None,
-            flags: int = 0, workspace=None):
+            flags: int = 0, workspace=None, tuning=None):
     """Jaccard coefficient of every stored edge (``sygraph::jaccard``).
 
     ``weights``: float32[n] vertex weights, or None for the unweighted form.
     ``result``: float32[m] output (allocated when None). Returns ``result``.
+    ``tuning``: dict of ``jc_tuning`` fields (``jc_jaccard_tuned``), or None.
     """
     lib = load_jaccard_lib()
     if result is None:
         result = _empty_f32(graph.m, graph.offsets.device)
     g = graph.c_struct()
     ws_ptr, ws_bytes = (None, 0) if workspace is None else (workspace.data_ptr(), workspace.numel() * workspace.element_size())
-    _check(lib.jc_jaccard_ex(ctypes.byref(g), _ptr(weights), _ptr(result) if graph.m else None,
-                             EDGE_CENTRIC if edge_centric else VERTEX_CENTRIC, flags, ws_ptr, ws_bytes,
-                             _stream(stream)), "jc_jaccard")
+    algo = EDGE_CENTRIC if edge_centric else VERTEX_CENTRIC
+    out = _ptr(result) if graph.m else None
+    if tuning is None:
+        _check(lib.jc_jaccard_ex(ctypes.byref(g), _ptr(weights), out, algo, flags, ws_ptr, ws_bytes,
+                                 _stream(stream)), "jc_jaccard")
+    else:
+        t = JcTuning(**tuning)
+        _check(lib.jc_jaccard_tuned(ctypes.byref(g), _ptr(weights), out, algo, flags, ws_ptr, ws_bytes,
+                                    ctypes.byref(t), _stream(stream)), "jc_jaccard_tuned")
     return result
 
 

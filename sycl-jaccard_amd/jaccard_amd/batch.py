@@ -94,9 +94,7 @@ def make_batches(count: int = 4096, seed: int = 1, **kw):
     return bu, bw
 
 
-class JcTuning(ctypes.Structure):
-    _fields_ = [("small_max", ctypes.c_int32), ("small_blocks_per_cu", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 6)]
+JcTuning = api.JcTuning
 
 
 def _tuned(dg, weights, out, ws, tuning, stream):

@@ -90,6 +90,30 @@ def test_hub_multichunk(kind, cuda):
     assert_bits_equal(ecw, vcw, "hub weighted EC vs VC")
 
 
+@pytest.mark.parametrize("name", ["rmat12_ef32", "K130", "hub_wide"])
+def test_row_load_paths_agree(name, cuda):
+    """Rows are streamed with 32-bit buffer offsets (index arrays below 2^30
+    entries) or with 64-bit addresses; forcing the latter (wide_index) must
+    give the same bits, both equal to the oracle."""
+    g = cases.sparse_hub_graph() if name == "hub_wide" else SUITE[name]
+    print(f"[gpu] load paths {name} n={g.n} m={g.m}", flush=True)
+    dg = api.DeviceCSR.from_host(g, cuda)
+    ref = oracle.ec_unweighted(g)
+    rng = np.random.default_rng(5)
+    v = (rng.integers(1, 1 << 24, g.n) / float(1 << 24)).astype(np.float32)
+    W = __import__("torch").from_numpy(v).to(cuda)
+    refw = oracle.weighted_f64(g, v.astype(np.float64))
+    for ec in (True, False):
+        narrow = api.jaccard(dg, edge_centric=ec).cpu().numpy()
+        wide = api.jaccard(dg, edge_centric=ec, tuning={"wide_index": 1}).cpu().numpy()
+        assert_bits_equal(narrow, ref, f"{name} ec={ec} buffer loads")
+        assert_bits_equal(wide, ref, f"{name} ec={ec} 64-bit loads")
+        narrow_w = api.jaccard(dg, weights=W, edge_centric=ec).cpu().numpy()
+        wide_w = api.jaccard(dg, weights=W, edge_centric=ec, tuning={"wide_index": 1}).cpu().numpy()
+        assert_bits_equal(wide_w, narrow_w, f"{name} ec={ec} weighted, load paths")
+        assert rel_err(narrow_w, refw).max() <= WEIGHTED_RTOL
+
+
 @pytest.mark.parametrize("ec", [True, False])
 def test_asymmetric_fixup(ec, cuda):
     g = cases.random_directed()

@@ -91,7 +91,13 @@ __device__ __forceinline__ bool exact_f64(const EngineArgs& a) {
 // ---------------------------------------------------------------------------
 // Bucketized LDS hash table: 2^lb buckets of kBucket int32 slots. Slots 0..6
 // hold keys (-1 = empty); slot 7 is the spill marker, set when a key had to
-// move on to the next bucket. Weighted tables keep the fp32 weight of key
+// move on to the next bucket. The slots are stored as two arrays of 16-byte
+// halves: slots 0..3 of bucket b at key[4b], slots 4..7 at key[hofs + 4b]
+// (hofs = 4 * the table's bucket capacity, a constant of the kernel, so the
+// second read's offset is an immediate). Consecutive buckets' halves are
+// consecutive 16-byte words: the two ds_read_b128 of a probe each spread
+// over every bank group of the LDS, where adjacent 32-byte buckets would put
+// each read on half of them. Weighted tables keep the fp32 weight of key
 // slot i of bucket b at val[8b + i]; val[7] (bucket 0's unused slot 7) holds
 // 0.0f and is what a missing lane reads, so every lane issues the same one
 // weight read without a branch and the missing lanes share one LDS word
@@ -128,9 +134,17 @@ struct Table {
   float* val;
   int lb;           // log2 of the bucket count
   uint32_t bmask;   // buckets - 1
+  int hofs;         // ints from the first half array to the second
   __device__ __forceinline__ uint32_t bucket(int32_t y) const { return hash_bucket(y, lb); }
   __device__ __forceinline__ uint32_t next(uint32_t b) const { return (b + 1) & bmask; }
-  __device__ __forceinline__ void clear(int i) const { key[i] = -1; }
+  __device__ __forceinline__ int32_t* slot_ptr(uint32_t b, int s) const {
+    return key + (s < 4 ? 0 : hofs) + 4 * b + (s & 3);
+  }
+  // clears the i-th 4-slot word of both halves, 0 <= i < 4 << lb
+  __device__ __forceinline__ void clear(int i) const {
+    key[i] = -1;
+    key[hofs + i] = -1;
+  }
   // the miss word; written once per table build, before any probe
   __device__ __forceinline__ void set_zero() const {
     if constexpr (W) val[kBucketKeys] = 0.0f;
@@ -138,16 +152,15 @@ struct Table {
   __device__ __forceinline__ void insert(int32_t y, float v) const {
     uint32_t b = bucket(y);
     for (;;) {
-      int32_t* kb = key + (b * kBucket);
       for (int s = 0; s < kBucketKeys; ++s) {
-        const int32_t old = atomicCAS(&kb[s], -1, y);
+        const int32_t old = atomicCAS(slot_ptr(b, s), -1, y);
         if (old == -1) {
           if constexpr (W) val[b * kBucket + s] = v;
           return;
         }
         if (old == y) return;
       }
-      kb[kBucketKeys] = kSpill;
+      *slot_ptr(b, kBucketKeys) = kSpill;
       b = next(b);
     }
   }
@@ -156,9 +169,8 @@ struct Table {
   // index from three masks: bit 0 = slots 1, 3, 5; bit 1 = 2, 3, 6; bit 2 =
   // 4, 5, 6) or at the miss word.
   __device__ __forceinline__ Hit look(uint32_t b, int32_t y) const {
-    const int4* kp = reinterpret_cast<const int4*>(key + b * kBucket);
-    const int4 k0 = kp[0];
-    const int4 k1 = kp[1];
+    const int4 k0 = *reinterpret_cast<const int4*>(key + 4 * b);
+    const int4 k1 = *reinterpret_cast<const int4*>(key + hofs + 4 * b);
     const bool e0 = k0.x == y, e1 = k0.y == y, e2 = k0.z == y, e3 = k0.w == y;
     const bool e4 = k1.x == y, e5 = k1.y == y, e6 = k1.z == y;
     Hit h;
@@ -796,8 +808,8 @@ __global__ __launch_bounds__(BLOCK) void k_owner_small(EngineArgs a) {
     if ((ballot(c0 && owns(u, du, x0, dx0)) | ballot(c1 && owns(u, du, x1, dx1))) == 0) continue;
     const int lb = table_log_buckets<W>(du);
     Chunk<W> cx;
-    cx.tab = Table<W>{key_all[w], W ? val_all[W ? w : 0] : nullptr, lb, (uint32_t)((1 << lb) - 1)};
-    for (int i = lane; i < (kBucket << lb); i += kWave) cx.tab.clear(i);
+    cx.tab = Table<W>{key_all[w], W ? val_all[W ? w : 0] : nullptr, lb, (uint32_t)((1 << lb) - 1), SLOTS / 2};
+    for (int i = lane; i < (4 << lb); i += kWave) cx.tab.clear(i);
     if (lane == 0) cx.tab.set_zero();
     lds_fence();
     build_table<W>(cx.tab, a.ind, a.vw, ou, du, lane, kWave);
@@ -1043,8 +1055,8 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
       const int clen = ce - cs;
       const int lb = table_log_buckets<W>(clen);
       Chunk<W> cx;
-      cx.tab = Table<W>{tkey, tval, lb, (uint32_t)((1 << lb) - 1)};
-      for (int i = tid; i < (kBucket << lb); i += BLOCK) cx.tab.clear(i);
+      cx.tab = Table<W>{tkey, tval, lb, (uint32_t)((1 << lb) - 1), Cfg::NB * 4};
+      for (int i = tid; i < (4 << lb); i += BLOCK) cx.tab.clear(i);
       if (tid == 0) {
         s_next = 0;
         cx.tab.set_zero();

@@ -42,12 +42,6 @@ __device__ __forceinline__ DD2 add_d(DD2 a, double v) {
   return DD2{hi, t - (hi - s)};
 }
 
-__device__ __forceinline__ double wave_reduce_dd(DD2 v) {
-#pragma unroll
-  for (int s = 1; s < kWave; s <<= 1) v = add_dd(v, DD2{__shfl_xor(v.hi, s), __shfl_xor(v.lo, s)});
-  return v.hi;
-}
-
 int grid_for(int64_t items, int per_block, int cap_per_cu) {
   const int64_t g = (items + per_block - 1) / per_block;
   return (int)std::max<int64_t>(1, std::min<int64_t>(g, (int64_t)device_cu_count() * cap_per_cu));
@@ -95,9 +89,10 @@ __global__ void k_rowsum_deg(int32_t n, const int32_t* __restrict__ off, float* 
 
 // Weighted row sum over column weights: sum_{k in row u} v[ind[k]] (ind may be
 // nullptr: identity index, i.e. the row's own edge weights, as main.cpp:216-257
-// arranges with its presumInd buffer). Rows up to kRowWave entries: one wave
-// per row; longer rows: one workgroup per row (second kernel), so a hub row is
-// not a serial tail. Loads are unrolled 4x per lane.
+// arranges with its presumInd buffer). Rows up to kRowWave entries: groups
+// of 8 lanes or whole waves (k_rowsum_w); longer rows: one workgroup per row
+// (second kernel), so a hub row is not a serial tail. Loads are unrolled 4x
+// per lane.
 #ifndef JC_ROW_WAVE
 #define JC_ROW_WAVE 2048
 #endif
@@ -124,22 +119,61 @@ __device__ __forceinline__ void gather_step(const int32_t* __restrict__ ind, con
     if (p0 + t * stride + lane >= end) x[t] = 0.0f;
 }
 
+// Rows up to kRowWave entries, 8 consecutive rows per wave and iteration:
+// a row of at most kRowShort entries (most rows of power-law graphs) is
+// summed by its own group of 8 lanes; the longer ones of the 8 are then
+// summed one after another by the whole wave. (One wave per row leaves most
+// lanes idle and pays a row's load latency per row.) The double-double sums
+// are exact, so the lane split does not change the result.
+constexpr int kRowShort = 64;
+constexpr int kRowGroup = 8;
+
 template <typename OutT, bool IDX>
 __global__ void k_rowsum_w(int32_t n, const int32_t* __restrict__ off, const int32_t* __restrict__ ind,
                            const float* __restrict__ v, OutT* __restrict__ out) {
+  constexpr int R = kWave / kRowGroup;  // rows per wave and iteration
+  const int lane = lane_id();
+  const int gl = lane % kRowGroup;
   const int nw = gridDim.x * (blockDim.x / kWave);
-  for (int32_t u = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; u < n; u += nw) {
-    const int o = off[u], d = off[u + 1] - o;
-    if (d > kRowWave) continue;
-    DD2 acc{0.0, 0.0};
-    for (int p0 = o; p0 < o + d; p0 += kWave * kRowU) {
-      float x[kRowU];
-      gather_step<IDX, kRowU>(ind, v, p0, kWave, lane_id(), o + d, x);
-#pragma unroll
-      for (int t = 0; t < kRowU; ++t) acc = add_d(acc, (double)x[t]);
+  for (int32_t base = (blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * R; base < n; base += nw * R) {
+    const int32_t u = base + lane / kRowGroup;
+    int o = 0, d = 0;
+    if (u < n) {
+      o = off[u];
+      d = off[u + 1] - o;
     }
-    const double s = wave_reduce_dd(acc);
-    if (lane_id() == 0) out[u] = (OutT)s;
+    const bool short_row = u < n && d <= kRowShort;
+    {
+      const int ds = short_row ? d : 0;
+      DD2 acc{0.0, 0.0};
+      for (int p0 = o; p0 < o + ds; p0 += kRowGroup * kRowU) {
+        float x[kRowU];
+        gather_step<IDX, kRowU>(ind, v, p0, kRowGroup, gl, o + ds, x);
+#pragma unroll
+        for (int t = 0; t < kRowU; ++t) acc = add_d(acc, (double)x[t]);
+      }
+#pragma unroll
+      for (int sh = 1; sh < kRowGroup; sh <<= 1)
+        acc = add_dd(acc, DD2{__shfl_xor(acc.hi, sh), __shfl_xor(acc.lo, sh)});
+      if (short_row && gl == 0) out[u] = (OutT)acc.hi;
+    }
+    uint64_t med = __ballot(gl == 0 && u < n && d > kRowShort && d <= kRowWave);
+    while (med) {
+      const int src = __ffsll((unsigned long long)med) - 1;
+      med &= med - 1;
+      const int32_t uu = __shfl(u, src);
+      const int oo = __shfl(o, src), dd = __shfl(d, src);
+      DD2 acc{0.0, 0.0};
+      for (int p0 = oo; p0 < oo + dd; p0 += kWave * kRowU) {
+        float x[kRowU];
+        gather_step<IDX, kRowU>(ind, v, p0, kWave, lane, oo + dd, x);
+#pragma unroll
+        for (int t = 0; t < kRowU; ++t) acc = add_d(acc, (double)x[t]);
+      }
+#pragma unroll
+      for (int sh = 1; sh < kWave; sh <<= 1) acc = add_dd(acc, DD2{__shfl_xor(acc.hi, sh), __shfl_xor(acc.lo, sh)});
+      if (lane == 0) out[uu] = (OutT)acc.hi;
+    }
   }
 }
 
@@ -211,12 +245,13 @@ static hipError_t rowsum_both(int32_t n, const int32_t* off, const int32_t* ind,
   const int cus = device_cu_count();
   hipLaunchKernelGGL(k_fill<uint32_t>, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t*>(counters), (int64_t)2, 0u);
   hipLaunchKernelGGL(k_collect_long, dim3(grid_for(n, 256, 8)), dim3(256), 0, s, n, off, list, counters);
+  const int gs = grid_rows((n + kWave / kRowGroup - 1) / (kWave / kRowGroup), 4, 64);
   if (ind) {
-    hipLaunchKernelGGL((k_rowsum_w<OutT, true>), dim3(grid_rows(n, 4, 64)), dim3(256), 0, s, n, off, ind, v, out);
+    hipLaunchKernelGGL((k_rowsum_w<OutT, true>), dim3(gs), dim3(256), 0, s, n, off, ind, v, out);
     hipLaunchKernelGGL((k_rowsum_w_long<OutT, true, 1024>), dim3(cus * 2), dim3(1024), 0, s, off, ind, v, out,
                        list, counters);
   } else {
-    hipLaunchKernelGGL((k_rowsum_w<OutT, false>), dim3(grid_rows(n, 4, 64)), dim3(256), 0, s, n, off, ind, v, out);
+    hipLaunchKernelGGL((k_rowsum_w<OutT, false>), dim3(gs), dim3(256), 0, s, n, off, ind, v, out);
     hipLaunchKernelGGL((k_rowsum_w_long<OutT, false, 1024>), dim3(cus * 2), dim3(1024), 0, s, off, ind, v, out,
                        list, counters);
   }

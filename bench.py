@@ -43,15 +43,20 @@ WORKLOADS = {
 TRAFFIC_PROFILES = {"rmat22_weighted": "profiles/owner_block_rmat22_weighted_pmc.md"}
 
 
-def measured_traffic(workload):
-    """HBM bytes per k_owner_block launch from the committed counter summary, or None."""
+def profile_value(workload, key, cast=float):
+    """A `key = value` line of the committed counter summary of the workload, or None."""
     rel = TRAFFIC_PROFILES.get(workload)
     if rel is None or not os.path.exists(os.path.join(ROOT, rel)):
         return None
     for line in open(os.path.join(ROOT, rel)):
-        if line.startswith("traffic_bytes_per_launch"):
-            return int(line.split("=")[1])
+        if line.startswith(key + " ="):
+            return cast(line.split("=")[1])
     return None
+
+
+def measured_traffic(workload):
+    """HBM bytes per k_owner_block launch from the committed counter summary, or None."""
+    return profile_value(workload, "traffic_bytes_per_launch", int)
 
 
 def make_graph(spec, weighted):
@@ -62,11 +67,16 @@ def make_graph(spec, weighted):
     return graphs.chung_lu(n=spec["n"], seed=spec["seed"], weighted=weighted)
 
 
-def algorithmic_bytes(g, st, weighted):
-    b = 4 * st["sum_min_deg"] + 4 * (g.n + 1) + 4 * g.m + 4 * g.m
+def compulsory_bytes(g, weighted):
+    """CSR (and weights) read once plus the output written once."""
+    b = 4 * (g.n + 1) + 4 * g.m + 4 * g.m
     if weighted:
         b += 4 * g.m + 8 * g.n
     return int(b)
+
+
+def algorithmic_bytes(g, st, weighted):
+    return int(4 * st["sum_min_deg"]) + compulsory_bytes(g, weighted)
 
 
 def cpu_baseline(g, budget_s=15.0, seed=0):
@@ -300,9 +310,24 @@ def main():
     print(f"[bench] {ms_per_step:.2f} ms/step, device {dev_ms:.2f} ms/step, owner_block "
           f"{(tb / nb if nb else 0):.2f} ms", file=sys.stderr, flush=True)
 
+    # one C-ABI call at a time, host clock around call + synchronize
+    calls = []
+    for _ in range(3):
+        t1 = time.perf_counter()
+        api.jaccard(dg, out, weights=W if weighted else None, edge_centric=ec, stream=stream)
+        torch.cuda.synchronize()
+        calls.append((time.perf_counter() - t1) * 1e3)
+    traffic = measured_traffic(args.workload)
+    ob_ms = tb / nb if nb else None
+    Bc = compulsory_bytes(g, weighted)
     extras = {"device_ms_per_step": dev_ms,
               "kernel_ms_per_step": {k: t / args.steps for k, (c, t) in kern.items()},
               "step_roofline_frac": B / (dev_ms * 1e-3) / HBM_PEAK if dev_ms else None,
+              "call_ms": sorted(calls)[1],
+              "compulsory_bytes": Bc,
+              "compulsory_frac": Bc / (dev_ms * 1e-3) / HBM_PEAK if dev_ms else None,
+              "traffic_achieved_gbs": traffic / (ob_ms * 1e-3) / 1e9 if (traffic and ob_ms) else None,
+              "tcc_hit_rate": profile_value(args.workload, "tcc_hit_rate"),
               "sum_min_deg": st["sum_min_deg"], "max_deg": st["max_deg"]}
     if args.extras:
         for name, kw in (("vc", dict(edge_centric=False)), ("ec_unweighted", dict(edge_centric=True))):

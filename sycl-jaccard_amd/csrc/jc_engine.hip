@@ -106,6 +106,9 @@ __device__ __forceinline__ bool exact_f64(const EngineArgs& a) {
 #ifndef JC_F64FAST
 #define JC_F64FAST 1
 #endif
+#ifndef JC_FETCHALL
+#define JC_FETCHALL 1
+#endif
 #ifndef JC_MIRRORCOUNT
 #define JC_MIRRORCOUNT 1
 #endif
@@ -168,9 +171,17 @@ struct Table {
   // compares; weighted tables add one ds_read_b32 at the hit slot (slot
   // index from three masks: bit 0 = slots 1, 3, 5; bit 1 = 2, 3, 6; bit 2 =
   // 4, 5, 6) or at the miss word.
+  __device__ __forceinline__ void fetch(uint32_t b, int4& k0, int4& k1) const {
+    k0 = *reinterpret_cast<const int4*>(key + 4 * b);
+    k1 = *reinterpret_cast<const int4*>(key + hofs + 4 * b);
+  }
   __device__ __forceinline__ Hit look(uint32_t b, int32_t y) const {
-    const int4 k0 = *reinterpret_cast<const int4*>(key + 4 * b);
-    const int4 k1 = *reinterpret_cast<const int4*>(key + hofs + 4 * b);
+    int4 k0, k1;
+    fetch(b, k0, k1);
+    return match(b, y, k0, k1);
+  }
+  // the compares and the weight read of look, on keys already fetched
+  __device__ __forceinline__ Hit match(uint32_t b, int32_t y, const int4& k0, const int4& k1) const {
     const bool e0 = k0.x == y, e1 = k0.y == y, e2 = k0.z == y, e3 = k0.w == y;
     const bool e4 = k1.x == y, e5 = k1.y == y, e6 = k1.z == y;
     Hit h;
@@ -451,16 +462,35 @@ __device__ __forceinline__ void probe_bm(const Chunk<W>& cx, const int32_t (&y)[
 // weight sums are known to be exact in plain fp64 (see ExactInfo), so hits
 // are added with one fp64 add instead of a double-double add; the result is
 // the same.
-template <bool W, int U, bool FAST>
+//
+// FA: all buckets of the pass are read before the first compare (one LDS
+// round trip for the keys of a pass instead of one per probe, for more
+// registers); the workgroup kernel uses it, the one-wave-per-row kernel,
+// whose occupancy the extra registers would cut, does not.
+template <bool W, int U, bool FAST, bool FA>
 __device__ __forceinline__ void probe_u(const Chunk<W>& cx, const int32_t (&y)[U], int& cnt, DD& acc) {
   uint32_t bk[U];
   Hit h[U];
   bool anyneed = false;
+  if constexpr (FA) {
+    int4 ka[U], kb[U];
 #pragma unroll
-  for (int t = 0; t < U; ++t) {
-    bk[t] = cx.tab.bucket(y[t]);
-    h[t] = cx.tab.look(bk[t], y[t]);
-    anyneed |= h[t].spill && !h[t].hit;
+    for (int t = 0; t < U; ++t) {
+      bk[t] = cx.tab.bucket(y[t]);
+      cx.tab.fetch(bk[t], ka[t], kb[t]);
+    }
+#pragma unroll
+    for (int t = 0; t < U; ++t) {
+      h[t] = cx.tab.match(bk[t], y[t], ka[t], kb[t]);
+      anyneed |= h[t].spill && !h[t].hit;
+    }
+  } else {
+#pragma unroll
+    for (int t = 0; t < U; ++t) {
+      bk[t] = cx.tab.bucket(y[t]);
+      h[t] = cx.tab.look(bk[t], y[t]);
+      anyneed |= h[t].spill && !h[t].hit;
+    }
   }
   if (ballot(anyneed)) {
 #pragma unroll
@@ -485,7 +515,7 @@ __device__ __forceinline__ void probe_u(const Chunk<W>& cx, const int32_t (&y)[U
 
 // Probes the U entries of one pass per lane (bitmap or hash table). COUNT:
 // also counts the entries below u and whether u occurs (mirror position).
-template <bool W, int G, int U, bool FAST, bool COUNT, bool STOP, bool BM>
+template <bool W, int G, int U, bool FAST, bool COUNT, bool STOP, bool BM, bool FA>
 __device__ __forceinline__ void probe_pass(const Chunk<W>& cx, int32_t (&y)[U], int& cnt, DD& acc, int& below,
                                            int& found) {
   if constexpr (COUNT) {
@@ -498,7 +528,7 @@ __device__ __forceinline__ void probe_pass(const Chunk<W>& cx, int32_t (&y)[U], 
   if constexpr (BM)
     probe_bm<W, U, FAST>(cx, y, cnt, acc);
   else
-    probe_u<W, U, FAST>(cx, y, cnt, acc);
+    probe_u<W, U, FAST, FA>(cx, y, cnt, acc);
 }
 
 // STOP: position of the first entry of the pass at p0 above cx.hi, or -1 when
@@ -556,7 +586,7 @@ __device__ __forceinline__ void load_pass(const EngineArgs& a, int32_t (&y)[U], 
 // pass that crosses it and return the position of the first entry above
 // cx.hi (entries of that pass beyond it are probed too, and miss). Returns
 // the stop position, or e.
-template <bool W, int G, int U, bool FAST, bool COUNT, bool STOP, bool BM, bool ONEPASS, bool BUF>
+template <bool W, int G, int U, bool FAST, bool COUNT, bool STOP, bool BM, bool ONEPASS, bool BUF, bool FA>
 __device__ __forceinline__ int stream_t(const EngineArgs& a, const Chunk<W>& cx, int s, int e, int& cnt, DD& acc,
                                         int& below, int& found) {
   const int gl = lane_id() & (G - 1);
@@ -569,7 +599,7 @@ __device__ __forceinline__ int stream_t(const EngineArgs& a, const Chunk<W>& cx,
     const int pn = p0 + G * U;
     int32_t yn[U];
     load_pass<BUF, G, U>(a, yn, pn, e);
-    probe_pass<W, G, U, FAST, COUNT, STOP, BM>(cx, y, cnt, acc, below, found);
+    probe_pass<W, G, U, FAST, COUNT, STOP, BM, FA>(cx, y, cnt, acc, below, found);
     if constexpr (STOP) {
       const int st = pass_stop<G, U>(y, cx.hi, p0, e);
       if (st >= 0) return st;
@@ -581,7 +611,7 @@ __device__ __forceinline__ int stream_t(const EngineArgs& a, const Chunk<W>& cx,
 #pragma unroll
   for (int t = 0; t < U; ++t)
     if (p0 + t * G + gl >= e) y[t] = INT32_MAX;
-  probe_pass<W, G, U, FAST, COUNT, STOP, BM>(cx, y, cnt, acc, below, found);
+  probe_pass<W, G, U, FAST, COUNT, STOP, BM, FA>(cx, y, cnt, acc, below, found);
   if constexpr (STOP) {
     const int st = pass_stop<G, U>(y, cx.hi, p0, e);
     if (st >= 0) return st;
@@ -589,12 +619,12 @@ __device__ __forceinline__ int stream_t(const EngineArgs& a, const Chunk<W>& cx,
   return e;
 }
 
-template <bool W, int G, int U, bool FAST, bool COUNT, bool STOP, bool BM, bool ONEPASS = false>
+template <bool W, int G, int U, bool FAST, bool COUNT, bool STOP, bool BM, bool ONEPASS, bool FA>
 __device__ __forceinline__ int stream(const EngineArgs& a, const Chunk<W>& cx, int s, int e, int& cnt, DD& acc,
                                       int& below, int& found) {
   if (!a.wide_index)
-    return stream_t<W, G, U, FAST, COUNT, STOP, BM, ONEPASS, true>(a, cx, s, e, cnt, acc, below, found);
-  return stream_t<W, G, U, FAST, COUNT, STOP, BM, ONEPASS, false>(a, cx, s, e, cnt, acc, below, found);
+    return stream_t<W, G, U, FAST, COUNT, STOP, BM, ONEPASS, true, FA>(a, cx, s, e, cnt, acc, below, found);
+  return stream_t<W, G, U, FAST, COUNT, STOP, BM, ONEPASS, false, FA>(a, cx, s, e, cnt, acc, below, found);
 }
 
 // One unit of work: each group of G lanes handles one owned short row x
@@ -624,14 +654,15 @@ __device__ __forceinline__ void unit(const EngineArgs& a, const Chunk<W>& cx, bo
   // position); multi-chunk units locate u with one group search at the end
   constexpr bool kCountMirror = !MULTI && (JC_MIRRORCOUNT || G * U <= 64);
   constexpr bool kOnePass = G * U <= 64;
+  constexpr bool kFetchAll = JC_FETCHALL && !SMALL;  // probe_u
   int cnt = 0, below = 0, found = 0;
   DD acc{0.0, 0.0};
   int stop_at = e;
   if (MULTI && !last)
-    stop_at = stream<W, G, U, FAST, kCountMirror, true, BM>(a, cx, s, e, cnt, acc, below, found);
+    stop_at = stream<W, G, U, FAST, kCountMirror, true, BM, false, kFetchAll>(a, cx, s, e, cnt, acc, below, found);
   else
-    stream<W, G, U, FAST, kCountMirror, false, BM, SMALL || (!MULTI && kOnePass)>(a, cx, s, e, cnt, acc, below,
-                                                                                 found);
+    stream<W, G, U, FAST, kCountMirror, false, BM, SMALL || (!MULTI && kOnePass), kFetchAll>(a, cx, s, e, cnt, acc,
+                                                                                            below, found);
   (void)gm;
   (void)gl;
   if constexpr (kCountMirror && kOnePass) {
@@ -849,6 +880,16 @@ constexpr size_t kBlockLds = JC_BLOCK_LDS_KIB * 1024 - 512;  // minus the kernel
 #define JC_BITMAP 1
 #endif
 
+// Table kind of a long row, from its first and last column: the bitmap when
+// its words (weighted: with their ranks) and, weighted, the keys' weights and
+// the zero word fit the table region of the LDS. Used by the scheduler and
+// by the task kernels alike.
+template <bool W>
+__host__ __device__ __forceinline__ bool bitmap_fits(int32_t lo, int32_t hi, int du, size_t budget) {
+  const uint32_t words = ((uint32_t)(hi - lo) >> 5) + 1u;
+  return JC_BITMAP && (size_t)words * (W ? 8u : 4u) + (W ? (size_t)4 * (du + 1) : 0) <= budget;
+}
+
 // Exclusive prefix sum over the workgroup (all threads active).
 __device__ __forceinline__ int block_excl_scan(int v, int* s_wsum) {
   const int lane = lane_id(), w = threadIdx.x / kWave;
@@ -902,7 +943,10 @@ __device__ __forceinline__ void run_unit_lds(const EngineArgs& a, const Chunk<W>
   }
 }
 
-template <int MODE, bool W, int BLOCK>
+// KIND: the task list (and table) this instantiation serves, 0 hash table, 1
+// bitmap; separate instantiations keep the register allocation of one path
+// from constraining the other.
+template <int MODE, bool W, int BLOCK, int KIND>
 __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
   static_assert(BLOCK >= kTaskSpan, "one thread per task entry during compaction");
   using Cfg = BlockCfg<W>;
@@ -928,26 +972,26 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
   const int tid = threadIdx.x;
   const int lane = lane_id();
   const int w = tid / kWave;
-  const int ntasks = *a.num_tasks;
+  const int ntasks = a.num_tasks[KIND];
+  const int tbase = KIND == 0 ? 0 : a.num_tasks[0];
   const bool fast = exact_f64(a);
   for (;;) {
     __syncthreads();
-    if (tid == 0) s_task = atomicAdd(a.task_counter, 1);
+    if (tid == 0) s_task = atomicAdd(&a.task_counter[KIND], 1);
     __syncthreads();
     const int t = s_task;
     if (t >= ntasks) return;
-    const int2 tk = a.tasks[t];
+    const int2 tk = a.tasks[tbase + t];
     const int32_t u = tk.x;
     const int ou = a.off[u];
     const int du = a.off[u + 1] - ou;
     const int ka = ou + tk.y * kTaskSpan;
     const int span = min(kTaskSpan, ou + du - ka);
-    // bitmap table when the row's id span fits the table region of the LDS
+    // bitmap tasks: the row's id span fits the table region of the LDS
+    // (bitmap_fits, decided by the scheduler)
     const int32_t lo0 = a.ind[ou], hi0 = a.ind[ou + du - 1];
     const uint32_t words = ((uint32_t)(hi0 - lo0) >> 5) + 1u;
-    const bool bm = JC_BITMAP &&
-                    (size_t)words * (W ? 8u : 4u) + (W ? (size_t)4 * (du + 1) : 0) <= Cfg::key_bytes + Cfg::val_bytes;
-    const int nch = bm ? 1 : (du + CH - 1) / CH;
+    const int nch = KIND == 1 ? 1 : (du + CH - 1) / CH;
 
     // owned entries of the task, sorted into length classes (heaviest first)
     int32_t x = 0;
@@ -989,7 +1033,7 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
       ij[at] = (uint16_t)tid;
     }
     const int nunits = s_ubase[kClasses];
-    if (bm) {
+    if constexpr (KIND == 1) {
       // the words (weighted: interleaved with their ranks), then the keys'
       // weights in id order and the zero word
       constexpr uint32_t WS = W ? 2u : 1u;  // stride of the words
@@ -1049,49 +1093,51 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
       __syncthreads();
       continue;
     }
-    for (int c = 0; c < nch; ++c) {
-      const int cs = ou + c * CH;
-      const int ce = min(cs + CH, ou + du);
-      const int clen = ce - cs;
-      const int lb = table_log_buckets<W>(clen);
-      Chunk<W> cx;
-      cx.tab = Table<W>{tkey, tval, lb, (uint32_t)((1 << lb) - 1), Cfg::NB * 4};
-      for (int i = tid; i < (4 << lb); i += BLOCK) cx.tab.clear(i);
-      if (tid == 0) {
-        s_next = 0;
-        cx.tab.set_zero();
+    if constexpr (KIND == 0) {
+      for (int c = 0; c < nch; ++c) {
+        const int cs = ou + c * CH;
+        const int ce = min(cs + CH, ou + du);
+        const int clen = ce - cs;
+        const int lb = table_log_buckets<W>(clen);
+        Chunk<W> cx;
+        cx.tab = Table<W>{tkey, tval, lb, (uint32_t)((1 << lb) - 1), Cfg::NB * 4};
+        for (int i = tid; i < (4 << lb); i += BLOCK) cx.tab.clear(i);
+        if (tid == 0) {
+          s_next = 0;
+          cx.tab.set_zero();
+        }
+        __syncthreads();
+        build_table<W>(cx.tab, a.ind, a.vw, cs, clen, tid, BLOCK);
+        __syncthreads();
+        cx.u = u;
+        cx.du = du;
+        cx.su = row_sum_of<MODE>(a, u);
+        cx.lo = a.ind[cs];
+        cx.hi = a.ind[ce - 1];
+        cx.c = c;
+        cx.nch = nch;
+        cx.kbase = ka;
+        cx.cursor = cursor;
+        cx.pcnt = pcnt;
+        cx.pacc = pacc;
+        for (;;) {
+          int un = 0;
+          if (lane == 0) un = atomicAdd(&s_next, 1);
+          un = __shfl(un, 0);
+          if (un >= nunits) break;
+          int k = 0;
+          while (un >= s_ubase[k + 1]) ++k;  // class slot k (0 = heaviest)
+          const int cls_k = kClasses - 1 - k;
+          const int per = kWave / class_g(cls_k);
+          const int first = s_cbase[k] + (un - s_ubase[k]) * per;
+          const int cend = s_cbase[k + 1];
+          if (nch == 1)
+            run_unit_lds<MODE, W, false>(a, cx, cls_k, first, cend, ix, iox, idx_, ij, fast);
+          else
+            run_unit_lds<MODE, W, true>(a, cx, cls_k, first, cend, ix, iox, idx_, ij, fast);
+        }
+        __syncthreads();
       }
-      __syncthreads();
-      build_table<W>(cx.tab, a.ind, a.vw, cs, clen, tid, BLOCK);
-      __syncthreads();
-      cx.u = u;
-      cx.du = du;
-      cx.su = row_sum_of<MODE>(a, u);
-      cx.lo = a.ind[cs];
-      cx.hi = a.ind[ce - 1];
-      cx.c = c;
-      cx.nch = nch;
-      cx.kbase = ka;
-      cx.cursor = cursor;
-      cx.pcnt = pcnt;
-      cx.pacc = pacc;
-      for (;;) {
-        int un = 0;
-        if (lane == 0) un = atomicAdd(&s_next, 1);
-        un = __shfl(un, 0);
-        if (un >= nunits) break;
-        int k = 0;
-        while (un >= s_ubase[k + 1]) ++k;  // class slot k (0 = heaviest)
-        const int cls_k = kClasses - 1 - k;
-        const int per = kWave / class_g(cls_k);
-        const int first = s_cbase[k] + (un - s_ubase[k]) * per;
-        const int cend = s_cbase[k + 1];
-        if (nch == 1)
-          run_unit_lds<MODE, W, false>(a, cx, cls_k, first, cend, ix, iox, idx_, ij, fast);
-        else
-          run_unit_lds<MODE, W, true>(a, cx, cls_k, first, cend, ix, iox, idx_, ij, fast);
-      }
-      __syncthreads();
     }
   }
 }
@@ -1190,14 +1236,17 @@ __global__ __launch_bounds__(BLOCK) void k_fixup(EngineArgs a) {
 // workgroups take the heaviest rows first.
 constexpr int kBuckets = 32;
 
+// Two task lists, one per table kind (0: hash table, 1: bitmap), each in
+// degree-descending buckets; list 0 first in the task array.
 struct Sched {
-  int bucket_cnt[kBuckets];   // tasks per bucket
-  int bucket_cur[kBuckets];   // emission cursors
-  int num_tasks;
-  int task_counter;
+  int bucket_cnt[2][kBuckets];   // tasks per bucket
+  int bucket_cur[2][kBuckets];   // emission cursors
+  int num_tasks[2];
+  int task_counter[2];
   int num_small;
   int small_cur;
 };
+
 
 __device__ __forceinline__ int bucket_of(int d) { return 31 - __clz(d); }
 
@@ -1213,25 +1262,28 @@ __device__ __forceinline__ int32_t seg_row(const int2* __restrict__ segs, int ns
   return segs[lo].x + (v - segs[lo].y);
 }
 
-template <int BLOCK>
+template <bool W, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_sched_count(int32_t rows, const int32_t* __restrict__ off,
-                                                       Sched* sc, int small_max, const int2* segs, int nseg) {
-  __shared__ int hist[kBuckets + 1];
-  for (int i = threadIdx.x; i < kBuckets + 1; i += BLOCK) hist[i] = 0;
+                                                       const int32_t* __restrict__ ind, Sched* sc, int small_max,
+                                                       size_t bm_budget, const int2* segs, int nseg) {
+  __shared__ int hist[2 * kBuckets + 1];
+  for (int i = threadIdx.x; i < 2 * kBuckets + 1; i += BLOCK) hist[i] = 0;
   __syncthreads();
   for (int32_t v = blockIdx.x * BLOCK + threadIdx.x; v < rows; v += gridDim.x * BLOCK) {
     const int32_t u = seg_row(segs, nseg, v);
-    const int d = off[u + 1] - off[u];
+    const int o = off[u], d = off[u + 1] - o;
     if (d <= 0) continue;
-    if (d <= small_max)
-      atomicAdd(&hist[kBuckets], 1);
-    else
-      atomicAdd(&hist[bucket_of(d)], (d + kTaskSpan - 1) / kTaskSpan);
+    if (d <= small_max) {
+      atomicAdd(&hist[2 * kBuckets], 1);
+    } else {
+      const int kind = bitmap_fits<W>(ind[o], ind[o + d - 1], d, bm_budget) ? 1 : 0;
+      atomicAdd(&hist[kind * kBuckets + bucket_of(d)], (d + kTaskSpan - 1) / kTaskSpan);
+    }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < kBuckets; i += BLOCK)
-    if (hist[i]) atomicAdd(&sc->bucket_cnt[i], hist[i]);
-  if (threadIdx.x == 0 && hist[kBuckets]) atomicAdd(&sc->num_small, hist[kBuckets]);
+  for (int i = threadIdx.x; i < 2 * kBuckets; i += BLOCK)
+    if (hist[i]) atomicAdd(&sc->bucket_cnt[i / kBuckets][i % kBuckets], hist[i]);
+  if (threadIdx.x == 0 && hist[2 * kBuckets]) atomicAdd(&sc->num_small, hist[2 * kBuckets]);
 }
 
 __global__ void k_sched_init(Sched* sc) {
@@ -1242,46 +1294,55 @@ __global__ void k_sched_init(Sched* sc) {
 __global__ void k_sched_scan(Sched* sc) {
   if (threadIdx.x != 0) return;
   int acc = 0;
-  for (int b = kBuckets - 1; b >= 0; --b) {
-    sc->bucket_cur[b] = acc;
-    acc += sc->bucket_cnt[b];
+  for (int k = 0; k < 2; ++k) {
+    const int first = acc;
+    for (int b = kBuckets - 1; b >= 0; --b) {
+      sc->bucket_cur[k][b] = acc;
+      acc += sc->bucket_cnt[k][b];
+    }
+    sc->num_tasks[k] = acc - first;
+    sc->task_counter[k] = 0;
   }
-  sc->num_tasks = acc;
-  sc->task_counter = 0;
   sc->small_cur = 0;
 }
 
-template <int BLOCK>
+template <bool W, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_sched_emit(int32_t rows, const int32_t* __restrict__ off,
-                                                      Sched* sc, int2* tasks, int32_t* small_rows, int small_max,
+                                                      const int32_t* __restrict__ ind, Sched* sc, int2* tasks,
+                                                      int32_t* small_rows, int small_max, size_t bm_budget,
                                                       const int2* segs, int nseg) {
-  __shared__ int hist[kBuckets + 1];
-  __shared__ int basep[kBuckets + 1];
+  constexpr int kSmallBin = 2 * kBuckets;
+  __shared__ int hist[kSmallBin + 1];
+  __shared__ int basep[kSmallBin + 1];
   for (int32_t v0 = blockIdx.x * BLOCK; v0 < rows; v0 += gridDim.x * BLOCK) {
-    for (int i = threadIdx.x; i < kBuckets + 1; i += BLOCK) hist[i] = 0;
+    for (int i = threadIdx.x; i < kSmallBin + 1; i += BLOCK) hist[i] = 0;
     __syncthreads();
     const int32_t v = v0 + threadIdx.x;
     const int32_t u = v < rows ? seg_row(segs, nseg, v) : 0;
     int d = 0, b = -1, cntv = 0, local = 0;
     if (v < rows) {
-      d = off[u + 1] - off[u];
+      const int o = off[u];
+      d = off[u + 1] - o;
       if (d > 0) {
         if (d <= small_max) {
-          b = kBuckets;
+          b = kSmallBin;
           cntv = 1;
         } else {
-          b = bucket_of(d);
+          const int kind = bitmap_fits<W>(ind[o], ind[o + d - 1], d, bm_budget) ? 1 : 0;
+          b = kind * kBuckets + bucket_of(d);
           cntv = (d + kTaskSpan - 1) / kTaskSpan;
         }
         local = atomicAdd(&hist[b], cntv);
       }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < kBuckets + 1; i += BLOCK) {
-      if (hist[i]) basep[i] = i == kBuckets ? atomicAdd(&sc->small_cur, hist[i]) : atomicAdd(&sc->bucket_cur[i], hist[i]);
+    for (int i = threadIdx.x; i < kSmallBin + 1; i += BLOCK) {
+      if (hist[i])
+        basep[i] = i == kSmallBin ? atomicAdd(&sc->small_cur, hist[i])
+                                  : atomicAdd(&sc->bucket_cur[i / kBuckets][i % kBuckets], hist[i]);
     }
     __syncthreads();
-    if (b == kBuckets) {
+    if (b == kSmallBin) {
       small_rows[basep[b] + local] = u;
     } else if (b >= 0) {
       const int at = basep[b] + local;
@@ -1329,8 +1390,8 @@ static hipError_t run_engine_t(const EngineArgs& base, void* ws, bool fixup, hip
   p += align_up((size_t)max_tasks(rows, edges) * sizeof(int2), 256);
   int32_t* small = reinterpret_cast<int32_t*>(p);
   a.tasks = tasks;
-  a.num_tasks = &sc->num_tasks;
-  a.task_counter = &sc->task_counter;
+  a.num_tasks = sc->num_tasks;
+  a.task_counter = sc->task_counter;
   a.small_rows = small;
   a.num_small = &sc->num_small;
 
@@ -1340,27 +1401,35 @@ static hipError_t run_engine_t(const EngineArgs& base, void* ws, bool fixup, hip
   if ((err = hipGetLastError()) != hipSuccess) return err;
   if (rows > 0) {
     const int g = std::min<int64_t>((rows + PB - 1) / PB, (int64_t)cus * 8);
+    using Cfg = BlockCfg<W>;
+    const size_t bm_budget = Cfg::key_bytes + Cfg::val_bytes;
     {
       ProfScope ps("sched", stream);
-      hipLaunchKernelGGL(k_sched_count<PB>, dim3(g), dim3(PB), 0, stream, rows, a.off, sc, a.small_max, a.segs,
-                         a.nseg);
+      hipLaunchKernelGGL((k_sched_count<W, PB>), dim3(g), dim3(PB), 0, stream, rows, a.off, a.ind, sc, a.small_max,
+                         bm_budget, a.segs, a.nseg);
       hipLaunchKernelGGL(k_sched_scan, dim3(1), dim3(64), 0, stream, sc);
-      hipLaunchKernelGGL(k_sched_emit<PB>, dim3(g), dim3(PB), 0, stream, rows, a.off, sc, tasks, small, a.small_max,
-                         a.segs, a.nseg);
+      hipLaunchKernelGGL((k_sched_emit<W, PB>), dim3(g), dim3(PB), 0, stream, rows, a.off, a.ind, sc, tasks, small,
+                         a.small_max, bm_budget, a.segs, a.nseg);
       if ((err = hipGetLastError()) != hipSuccess) return err;
     }
 
-    using Cfg = BlockCfg<W>;
-    err = hipFuncSetAttribute(reinterpret_cast<const void*>(k_owner_block<MODE, W, BB>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)Cfg::lds_bytes);
+    // hash-table tasks, then bitmap tasks: one persistent grid each (a grid
+    // whose list is empty returns at once)
+    int occ0 = 0, occ1 = 0;
+    for (const void* f : {reinterpret_cast<const void*>(k_owner_block<MODE, W, BB, 0>),
+                          reinterpret_cast<const void*>(k_owner_block<MODE, W, BB, 1>)}) {
+      err = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)Cfg::lds_bytes);
+      if (err != hipSuccess) return err;
+    }
+    err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ0, k_owner_block<MODE, W, BB, 0>, BB, Cfg::lds_bytes);
     if (err != hipSuccess) return err;
-    int occ = 0;
-    err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_owner_block<MODE, W, BB>, BB, Cfg::lds_bytes);
+    err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ1, k_owner_block<MODE, W, BB, 1>, BB, Cfg::lds_bytes);
     if (err != hipSuccess) return err;
-    if (occ < 1) return hipErrorInvalidConfiguration;
+    if (occ0 < 1 || occ1 < 1) return hipErrorInvalidConfiguration;
     {
       ProfScope ps("owner_block", stream);
-      hipLaunchKernelGGL((k_owner_block<MODE, W, BB>), dim3(cus * occ), dim3(BB), Cfg::lds_bytes, stream, a);
+      hipLaunchKernelGGL((k_owner_block<MODE, W, BB, 0>), dim3(cus * occ0), dim3(BB), Cfg::lds_bytes, stream, a);
+      hipLaunchKernelGGL((k_owner_block<MODE, W, BB, 1>), dim3(cus * occ1), dim3(BB), Cfg::lds_bytes, stream, a);
       if ((err = hipGetLastError()) != hipSuccess) return err;
     }
     const int sg = std::min<int64_t>((rows + (SB / kWave) - 1) / (SB / kWave), (int64_t)cus * a.small_blocks_per_cu);

@@ -9,7 +9,9 @@ Inputs are resident in HBM before the timed region.
 
 Prints one JSON line. ``roofline`` uses the algorithmic bytes
   B = 4*sum_{stored (u,v)} min(d_u,d_v) + 4(n+1) + 4m + 4m  (+ 4m + 8n weighted)
-over the dominant kernel's average launch time (HIP events on its stream).
+over the dominant kernel's average time per call (HIP events on its stream
+around its two launches, the hash-table and the bitmap task lists of
+``k_owner_block``).
 ``cpu_baseline`` times the CPU restatement of the reference's edge-centric
 loop (binary search per shorter-row element) on a bounded random sample of
 the same graph's edges, with all host threads.

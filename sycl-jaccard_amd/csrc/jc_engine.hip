@@ -698,20 +698,29 @@ __device__ __forceinline__ void unit(const EngineArgs& a, const Chunk<W>& cx, bo
     if (!last) {
       if (valid && gl == 0) {
         cx.cursor[item] = stop_at;
-        if constexpr (W)
-          cx.pacc[item] = cx.c == 0 ? acc : dd_add(cx.pacc[item], acc);
-        else
+        if constexpr (W) {
+          // FAST: partial sums are exact in plain fp64 too
+          if constexpr (FAST)
+            cx.pacc[item] = DD{cx.c == 0 ? acc.hi : cx.pacc[item].hi + acc.hi, 0.0};
+          else
+            cx.pacc[item] = cx.c == 0 ? acc : dd_add(cx.pacc[item], acc);
+        } else {
           cx.pcnt[item] = cx.c == 0 ? (uint32_t)cnt : cx.pcnt[item] + (uint32_t)cnt;
+        }
       }
     } else {
       int q = end;
       if (valid) q = grp_lower_bound<G>(a.ind, ox, end, cx.u);
       if (valid && gl == 0) {
         uint32_t tot = (uint32_t)cnt;
-        if constexpr (W)
-          acc = dd_add(cx.pacc[item], acc);
-        else
+        if constexpr (W) {
+          if constexpr (FAST)
+            acc.hi += cx.pacc[item].hi;
+          else
+            acc = dd_add(cx.pacc[item], acc);
+        } else {
           tot += cx.pcnt[item];
+        }
         const bool f = q < end && a.ind[q] == cx.u;
         emit<MODE, W>(a, cx.u, cx.du, x, dx, cx.kbase + j, f ? (int64_t)q : -1, tot, acc, cx.su, sx);
       }

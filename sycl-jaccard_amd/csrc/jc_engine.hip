@@ -24,9 +24,11 @@
 //     before the current pass is probed.
 //   * Degree binning: rows with degree <= kSmallMax run one wave per row with
 //     a per-wave table; longer rows run one workgroup per (row, 896-entry
-//     piece) task from a degree-descending queue (persistent grid, atomic
-//     counter). A task sorts its owned entries into length classes, heaviest
-//     first, and its waves take class units from an LDS counter.
+//     piece) task from degree-descending queues (persistent grids, atomic
+//     counters), one queue and kernel instantiation per table kind (LDS
+//     bitmap when the row's id span fits, else hash table). A task sorts its
+//     owned entries into length classes, heaviest first, and its waves take
+//     class units from an LDS counter.
 //   * Rows longer than one LDS chunk are processed chunk by chunk; each short
 //     row resumes where the previous chunk stopped (no searches), its
 //     partial result kept in LDS.

@@ -206,6 +206,37 @@ def test_pairs(cuda):
     assert np.isnan(api.jaccard_list(dg, bad, ok).cpu().numpy()).all()
 
 
+def test_row_sums_at_length_boundaries(cuda):
+    """Row sums route rows by length (8 lanes up to 64 entries, a whole wave
+    up to 2048, a workgroup beyond); rows on both sides of each limit must
+    give the exact sums: vertex weights (edge weights of the row) and the
+    weighted RowSum (vertex weights gathered over the row's columns)."""
+    import torch
+    rng = np.random.default_rng(9)
+    degs = [0, 1, 7, 8, 9, 63, 64, 65, 66, 2047, 2048, 2049, 5000, 3, 0, 129]
+    n = 6000
+    rows = []
+    for r, d in enumerate(degs):
+        cols = rng.choice(np.setdiff1d(np.arange(n), [r]), size=d, replace=False)
+        rows.append(np.sort(cols).astype(np.int32))
+    rows += [np.zeros(0, np.int32)] * (n - len(degs))
+    off = np.zeros(n + 1, np.int64)
+    off[1:] = np.cumsum([r.size for r in rows])
+    ind = np.concatenate(rows).astype(np.int32)
+    w = (rng.integers(1, 1 << 24, ind.size) / float(1 << 24)).astype(np.float32)
+    g = graphs.CSRGraph(n=n, offsets=off.astype(np.int32), indices=ind, weights=w, name="row_lengths")
+    print(f"[gpu] row sums n={g.n} m={g.m}", flush=True)
+    dg = api.DeviceCSR.from_host(g, cuda)
+    W = api.vertex_weights(dg).cpu().numpy()
+    assert_bits_equal(W, oracle.vertex_weights(g).astype(np.float32), "vertex weights")
+    v = (rng.integers(1, 1 << 20, n) / float(1 << 20)).astype(np.float32)
+    work = torch.empty(n, dtype=torch.float32, device=cuda)
+    api.rowsum(dg, work, v=torch.from_numpy(v).to(cuda))
+    r_of = np.repeat(np.arange(n), np.diff(off))
+    S = np.bincount(r_of, weights=v[ind].astype(np.float64), minlength=n)
+    assert_bits_equal(work.cpu().numpy(), S.astype(np.float32), "weighted rowsum")
+
+
 def test_kernel_level_entry_points(cuda):
     import torch
     g = SUITE["rmat10"]

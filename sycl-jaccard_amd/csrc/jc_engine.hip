@@ -13,10 +13,13 @@
 //     search). Work is the sum over pairs of min(d_u, d_x): half of the
 //     reference's.
 //   * The owner's row is staged in LDS as a bucketized hash table: buckets
-//     of 8 keys (32 bytes, two ds_read_b128), at most two keys per bucket on
-//     average, weights in a parallel array. A probe is straight-line code;
-//     a full bucket (rare) sends the wave through a uniform slow path. The
-//     short row is streamed from HBM, several loads in flight per lane.
+//     of 8 slots (32 bytes as two 16-byte halves in separate arrays, two
+//     ds_read_b128), at most two keys per bucket on average weighted and
+//     three unweighted, weights in a parallel array; rows whose id span fits
+//     use an LDS bitmap instead. A probe is straight-line code; a full
+//     bucket (rare) sends the wave through a uniform slow path. The short
+//     row is streamed from HBM with raw buffer loads, several in flight per
+//     lane.
 //   * A wave is split into groups of G lanes, one short row per group, G and
 //     the unroll U chosen by the row's length class, so short rows do not
 //     leave lanes idle. Counts accumulate per lane (VALU) and are reduced

@@ -107,7 +107,12 @@ typedef struct jc_tuning {
                                   the index array is small enough for 32-bit buffer
                                   offsets (arrays of 2^30 entries or more always use
                                   them); for testing both load paths */
-  int32_t reserved[5];
+  int32_t bitmap_blocks_per_cu; /* workgroups per CU of the bitmap-table task kernel
+                                  (1 or 2, default 1). With 2 its LDS table region
+                                  shrinks to 64 KiB so two 1024-thread workgroups share
+                                  a CU; rows whose bitmap no longer fits use the hash
+                                  table kernel instead */
+  int32_t reserved[4];
 } jc_tuning;
 
 /* jc_jaccard_ex with launch tuning (NULL = defaults). */

@@ -95,6 +95,7 @@ int run_jaccard(const jc_csr* g, const float* vw, float* result, bool ec, unsign
   a.small_max = tuning ? tuning->small_max : 0;
   a.small_blocks_per_cu = tuning ? tuning->small_blocks_per_cu : 0;
   a.wide_index = tuning ? (tuning->wide_index != 0) : 0;
+  a.bitmap_blocks_per_cu = tuning ? tuning->bitmap_blocks_per_cu : 0;
   double* S64 = weighted ? reinterpret_cast<double*>(ws + L.s64) : nullptr;
   if (weighted) {
     jc::ExactInfo* ex = reinterpret_cast<jc::ExactInfo*>(ws + L.exact);

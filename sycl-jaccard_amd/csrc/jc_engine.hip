@@ -117,6 +117,9 @@ __device__ __forceinline__ bool exact_f64(const EngineArgs& a) {
 #ifndef JC_MIRRORCOUNT
 #define JC_MIRRORCOUNT 1
 #endif
+#ifndef JC_PINGPONG
+#define JC_PINGPONG 1  // row streaming loop unrolled twice (two register sets, no copies)
+#endif
 // c ? a : b per lane as one v_cndmask_b32 that the compiler cannot turn into
 // a branch (it would otherwise sink the computation of a into one).
 __device__ __forceinline__ uint32_t lane_select(bool c, uint32_t a, uint32_t b) {
@@ -419,25 +422,28 @@ struct Chunk {
   DD* pacc;
   // bitmap table (rows whose id span fits the LDS): bit (y - lo) is set for
   // every key y. Unweighted: word i at bits[i]. Weighted: word i at bits[2i]
-  // and the number of keys in the words before it at bits[2i + 1] (one
-  // 8-byte read per probe); bval[r] = weight of the r-th key in id order and
-  // bval[bzero] = 0.0f, the word a missing lane reads.
+  // and 1 + the number of keys in the words before it at bits[2i + 1] (one
+  // 8-byte read per probe); bval[1 + r] = weight of the r-th key in id order
+  // (as fp64, exact) and bval[0] = 0, the slot a missing lane reads. Word
+  // wzero (one past the last) is zero: ids outside the span read it.
   const uint32_t* bits;
-  const float* bval;
-  uint32_t span;    // hi - lo + 1
-  uint32_t bzero;
+  const double* bval;
+  uint32_t wzero;
 };
 
 // Probes U elements per lane against a bitmap table: one LDS read per probe
 // (weighted: the word and its rank together), then weighted tables read the
-// weight at rank + keys below the bit in the word, or the zero word.
+// weight at rank + keys below the bit in the word, or the zero word. An id
+// outside the row's span reads the zero guard word (ids below lo wrap to
+// large offsets); bits past the span in the last word are zero, so no range
+// test is needed.
 template <bool W, int U, bool FAST>
 __device__ __forceinline__ void probe_bm(const Chunk<W>& cx, const int32_t (&y)[U], int& cnt, DD& acc) {
   uint32_t off[U], wd[U], rk[U];
 #pragma unroll
   for (int t = 0; t < U; ++t) {
-    off[t] = (uint32_t)(y[t] - cx.lo);  // below lo wraps to a large value
-    const uint32_t wi = off[t] < cx.span ? off[t] >> 5 : 0u;
+    off[t] = (uint32_t)(y[t] - cx.lo);
+    const uint32_t wi = min(off[t] >> 5, cx.wzero);
     if constexpr (W) {
       const uint2 p = reinterpret_cast<const uint2*>(cx.bits)[wi];
       wd[t] = p.x;
@@ -449,16 +455,22 @@ __device__ __forceinline__ void probe_bm(const Chunk<W>& cx, const int32_t (&y)[
   }
 #pragma unroll
   for (int t = 0; t < U; ++t) {
-    const uint32_t sh = off[t] & 31u;
-    const bool hit = off[t] < cx.span && ((wd[t] >> sh) & 1u);
-    cnt += hit ? 1 : 0;
+    // bit (off & 31) of the word; v_bfe reads the low five bits of its
+    // offset and width operands
     if constexpr (W) {
-      const uint32_t r = rk[t] + __popc(wd[t] & ((1u << sh) - 1u));
-      const float v = cx.bval[hit ? r : cx.bzero];
+      // 0 or all ones
+      const uint32_t hitm = (uint32_t)__builtin_amdgcn_sbfe((int)wd[t], off[t], 1u);
+      cnt -= (int)hitm;
+      // slot 1 + rank of the key (keys of the word below the bit counted
+      // onto the word's stored 1 + rank), or slot 0 on a miss
+      const uint32_t r = __builtin_amdgcn_ubfe(wd[t], 0u, off[t]);
+      const double v = cx.bval[((uint32_t)__popc(r) + rk[t]) & hitm];
       if constexpr (FAST)
-        acc.hi += (double)v;
+        acc.hi += v;
       else
-        acc = dd_add1(acc, (double)v);
+        acc = dd_add1(acc, v);
+    } else {
+      cnt += (int)__builtin_amdgcn_ubfe(wd[t], off[t], 1u);
     }
   }
 }
@@ -519,15 +531,17 @@ __device__ __forceinline__ void probe_u(const Chunk<W>& cx, const int32_t (&y)[U
 }
 
 // Probes the U entries of one pass per lane (bitmap or hash table). COUNT:
-// also counts the entries below u and whether u occurs (mirror position).
+// also counts the entries below u (u's position in the row, the mirror
+// position) and collects the lanes that hold u in the wave mask `seen` (one
+// compare into a scalar mask per entry).
 template <bool W, int G, int U, bool FAST, bool COUNT, bool STOP, bool BM, bool FA>
 __device__ __forceinline__ void probe_pass(const Chunk<W>& cx, int32_t (&y)[U], int& cnt, DD& acc, int& below,
-                                           int& found) {
+                                           uint64_t& seen) {
   if constexpr (COUNT) {
 #pragma unroll
     for (int t = 0; t < U; ++t) {
       below += y[t] < cx.u ? 1 : 0;
-      found |= y[t] == cx.u ? 1 : 0;
+      seen |= ballot(y[t] == cx.u);
     }
   }
   if constexpr (BM)
@@ -593,30 +607,62 @@ __device__ __forceinline__ void load_pass(const EngineArgs& a, int32_t (&y)[U], 
 // the stop position, or e.
 template <bool W, int G, int U, bool FAST, bool COUNT, bool STOP, bool BM, bool ONEPASS, bool BUF, bool FA>
 __device__ __forceinline__ int stream_t(const EngineArgs& a, const Chunk<W>& cx, int s, int e, int& cnt, DD& acc,
-                                        int& below, int& found) {
+                                        int& below, uint64_t& seen) {
   const int gl = lane_id() & (G - 1);
   int p0 = s;
   if (p0 >= e) return e;
   int32_t y[U];
   load_pass<BUF, G, U>(a, y, p0, e);
-  // ONEPASS: the caller knows every row fits one pass (e - s <= G * U)
-  while (!ONEPASS && p0 + G * U < e) {  // the pass at p0 is full and another follows
-    const int pn = p0 + G * U;
-    int32_t yn[U];
-    load_pass<BUF, G, U>(a, yn, pn, e);
-    probe_pass<W, G, U, FAST, COUNT, STOP, BM, FA>(cx, y, cnt, acc, below, found);
-    if constexpr (STOP) {
-      const int st = pass_stop<G, U>(y, cx.hi, p0, e);
-      if (st >= 0) return st;
-    }
-    p0 = pn;
+  // ONEPASS: the caller knows every row fits one pass (e - s <= G * U).
+  // Otherwise the loop runs while the pass at p0 is full and another
+  // follows; it is unrolled twice so that the two register sets take turns
+  // (loads into one while the other is probed) without copies.
+  // unrolled for bitmap and weighted probes; measured slower for
+  // unweighted hash-table probes
+  constexpr bool kPingPong = JC_PINGPONG && (BM || W);
+  if constexpr (!ONEPASS && !kPingPong) {
+    while (p0 + G * U < e) {
+      int32_t yn[U];
+      load_pass<BUF, G, U>(a, yn, p0 + G * U, e);
+      probe_pass<W, G, U, FAST, COUNT, STOP, BM, FA>(cx, y, cnt, acc, below, seen);
+      if constexpr (STOP) {
+        const int st = pass_stop<G, U>(y, cx.hi, p0, e);
+        if (st >= 0) return st;
+      }
+      p0 += G * U;
 #pragma unroll
-    for (int t = 0; t < U; ++t) y[t] = yn[t];
+      for (int t = 0; t < U; ++t) y[t] = yn[t];
+    }
+  }
+  if constexpr (!ONEPASS && kPingPong) {
+    int32_t yb[U];
+    for (;;) {
+      if (p0 + G * U >= e) break;
+      load_pass<BUF, G, U>(a, yb, p0 + G * U, e);
+      probe_pass<W, G, U, FAST, COUNT, STOP, BM, FA>(cx, y, cnt, acc, below, seen);
+      if constexpr (STOP) {
+        const int st = pass_stop<G, U>(y, cx.hi, p0, e);
+        if (st >= 0) return st;
+      }
+      p0 += G * U;
+      if (p0 + G * U >= e) {
+#pragma unroll
+        for (int t = 0; t < U; ++t) y[t] = yb[t];
+        break;
+      }
+      load_pass<BUF, G, U>(a, y, p0 + G * U, e);
+      probe_pass<W, G, U, FAST, COUNT, STOP, BM, FA>(cx, yb, cnt, acc, below, seen);
+      if constexpr (STOP) {
+        const int st = pass_stop<G, U>(yb, cx.hi, p0, e);
+        if (st >= 0) return st;
+      }
+      p0 += G * U;
+    }
   }
 #pragma unroll
   for (int t = 0; t < U; ++t)
     if (p0 + t * G + gl >= e) y[t] = INT32_MAX;
-  probe_pass<W, G, U, FAST, COUNT, STOP, BM, FA>(cx, y, cnt, acc, below, found);
+  probe_pass<W, G, U, FAST, COUNT, STOP, BM, FA>(cx, y, cnt, acc, below, seen);
   if constexpr (STOP) {
     const int st = pass_stop<G, U>(y, cx.hi, p0, e);
     if (st >= 0) return st;
@@ -626,10 +672,10 @@ __device__ __forceinline__ int stream_t(const EngineArgs& a, const Chunk<W>& cx,
 
 template <bool W, int G, int U, bool FAST, bool COUNT, bool STOP, bool BM, bool ONEPASS, bool FA>
 __device__ __forceinline__ int stream(const EngineArgs& a, const Chunk<W>& cx, int s, int e, int& cnt, DD& acc,
-                                      int& below, int& found) {
+                                      int& below, uint64_t& seen) {
   if (!a.wide_index)
-    return stream_t<W, G, U, FAST, COUNT, STOP, BM, ONEPASS, true, FA>(a, cx, s, e, cnt, acc, below, found);
-  return stream_t<W, G, U, FAST, COUNT, STOP, BM, ONEPASS, false, FA>(a, cx, s, e, cnt, acc, below, found);
+    return stream_t<W, G, U, FAST, COUNT, STOP, BM, ONEPASS, true, FA>(a, cx, s, e, cnt, acc, below, seen);
+  return stream_t<W, G, U, FAST, COUNT, STOP, BM, ONEPASS, false, FA>(a, cx, s, e, cnt, acc, below, seen);
 }
 
 // One unit of work: each group of G lanes handles one owned short row x
@@ -660,26 +706,25 @@ __device__ __forceinline__ void unit(const EngineArgs& a, const Chunk<W>& cx, bo
   constexpr bool kCountMirror = !MULTI && (JC_MIRRORCOUNT || G * U <= 64);
   constexpr bool kOnePass = G * U <= 64;
   constexpr bool kFetchAll = JC_FETCHALL && !SMALL;  // probe_u
-  int cnt = 0, below = 0, found = 0;
+  int cnt = 0, below = 0;
+  uint64_t seen = 0;
   DD acc{0.0, 0.0};
   int stop_at = e;
   if (MULTI && !last)
-    stop_at = stream<W, G, U, FAST, kCountMirror, true, BM, false, kFetchAll>(a, cx, s, e, cnt, acc, below, found);
+    stop_at = stream<W, G, U, FAST, kCountMirror, true, BM, false, kFetchAll>(a, cx, s, e, cnt, acc, below, seen);
   else
     stream<W, G, U, FAST, kCountMirror, false, BM, SMALL || (!MULTI && kOnePass), kFetchAll>(a, cx, s, e, cnt, acc,
-                                                                                            below, found);
+                                                                                            below, seen);
   (void)gm;
   (void)gl;
   if constexpr (kCountMirror && kOnePass) {
-    // one pass of at most U entries per lane: the three counts fit a byte each
-    const int packed = grp_sum_i<G>(cnt | (below << 8) | (found << 16));
+    // one pass of at most U entries per lane: both counts fit a byte each
+    const int packed = grp_sum_i<G>(cnt | (below << 8));
     cnt = packed & 0xFF;
-    below = (packed >> 8) & 0xFF;
-    found = packed >> 16;
+    below = packed >> 8;
   } else if constexpr (kCountMirror) {
     cnt = grp_sum_i<G>(cnt);
     below = grp_sum_i<G>(below);
-    found = (ballot(found != 0) & gm) != 0 ? 1 : 0;
   } else {
     cnt = grp_sum_i<G>(cnt);
   }
@@ -692,7 +737,8 @@ __device__ __forceinline__ void unit(const EngineArgs& a, const Chunk<W>& cx, bo
   if constexpr (!MULTI) {
     int64_t q = -1;
     if constexpr (kCountMirror) {
-      if (found) q = (int64_t)ox + below;
+      // u sits at entry `below` of row x if a lane of the group saw it
+      if ((seen & gm) != 0) q = (int64_t)ox + below;
     } else {
       int pos = end;
       if (valid) pos = grp_lower_bound<G>(a.ind, ox, end, cx.u);
@@ -898,13 +944,13 @@ constexpr size_t kBlockLds = JC_BLOCK_LDS_KIB * 1024 - 512;  // minus the kernel
 #endif
 
 // Table kind of a long row, from its first and last column: the bitmap when
-// its words (weighted: with their ranks) and, weighted, the keys' weights and
-// the zero word fit the table region of the LDS. Used by the scheduler and
+// its words and the guard word (weighted: with their ranks) and, weighted,
+// the zero slot and the keys' fp64 weights fit the table region of the LDS. Used by the scheduler and
 // by the task kernels alike.
 template <bool W>
 __host__ __device__ __forceinline__ bool bitmap_fits(int32_t lo, int32_t hi, int du, size_t budget) {
-  const uint32_t words = ((uint32_t)(hi - lo) >> 5) + 1u;
-  return JC_BITMAP && (size_t)words * (W ? 8u : 4u) + (W ? (size_t)4 * (du + 1) : 0) <= budget;
+  const uint32_t words = ((uint32_t)(hi - lo) >> 5) + 1u;  // plus one zero guard word
+  return JC_BITMAP && (size_t)(words + 1u) * (W ? 8u : 4u) + (W ? (size_t)8 * (du + 1) : 0) <= budget;
 }
 
 // Exclusive prefix sum over the workgroup (all threads active).
@@ -923,6 +969,10 @@ __device__ __forceinline__ int block_excl_scan(int v, int* s_wsum) {
   __syncthreads();
   return base + incl - v;
 }
+
+// Table region of the bitmap-task kernel when two workgroups share a CU:
+// 2 x (64 KiB + item list + static LDS) fits the 160 KiB of a CU.
+constexpr size_t kBitmapLds2 = 64 * 1024;
 
 template <bool W>
 struct BlockCfg {
@@ -969,15 +1019,21 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
   using Cfg = BlockCfg<W>;
   constexpr int CH = Cfg::CH;
   constexpr int NW = BLOCK / kWave;
+  // LDS: the table region, then the item list; hash-table tasks add the
+  // multi-chunk cursors and partials after it. Bitmap tasks are single-chunk,
+  // and their table region is a.bm_budget bytes (the launcher sizes the
+  // dynamic LDS of this kind to it plus the item list, so a smaller budget
+  // lets two workgroups share a CU).
   extern __shared__ __align__(16) unsigned char smem[];
   int32_t* tkey = reinterpret_cast<int32_t*>(smem);
   float* tval = reinterpret_cast<float*>(smem + Cfg::key_bytes);
-  int32_t* ix = reinterpret_cast<int32_t*>(smem + Cfg::key_bytes + Cfg::val_bytes);
+  unsigned char* items = smem + (KIND == 1 ? (size_t)a.bm_budget : Cfg::key_bytes + Cfg::val_bytes);
+  int32_t* ix = reinterpret_cast<int32_t*>(items);
   int32_t* iox = ix + kTaskSpan;
   int32_t* idx_ = iox + kTaskSpan;
   uint16_t* ij = reinterpret_cast<uint16_t*>(idx_ + kTaskSpan);
-  int* cursor = reinterpret_cast<int*>(smem + Cfg::key_bytes + Cfg::val_bytes + Cfg::item_bytes);
-  unsigned char* partp = smem + Cfg::key_bytes + Cfg::val_bytes + Cfg::item_bytes + Cfg::cursor_bytes;
+  int* cursor = KIND == 1 ? nullptr : reinterpret_cast<int*>(items + Cfg::item_bytes);
+  unsigned char* partp = KIND == 1 ? nullptr : items + Cfg::item_bytes + Cfg::cursor_bytes;
   uint32_t* pcnt = reinterpret_cast<uint32_t*>(partp);
   DD* pacc = reinterpret_cast<DD*>(partp);
   __shared__ int s_task;
@@ -1051,21 +1107,23 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
     }
     const int nunits = s_ubase[kClasses];
     if constexpr (KIND == 1) {
-      // the words (weighted: interleaved with their ranks), then the keys'
-      // weights in id order and the zero word
+      // the words and the zero guard word (weighted: interleaved with
+      // 1 + their ranks), then, weighted, the zero slot and the keys'
+      // weights in id order as fp64
       constexpr uint32_t WS = W ? 2u : 1u;  // stride of the words
       uint32_t* bits = reinterpret_cast<uint32_t*>(smem);
-      float* bval = reinterpret_cast<float*>(bits + WS * words);
-      for (uint32_t i = tid; i < WS * words; i += BLOCK) bits[i] = 0u;
+      double* bval = reinterpret_cast<double*>(bits + WS * (words + 1u));
+      // words plus the zero guard word (weighted: with its rank slot)
+      for (uint32_t i = tid; i < WS * (words + 1u); i += BLOCK) bits[i] = 0u;
       if (tid == 0) s_next = 0;
       __syncthreads();
       for (int i = tid; i < du; i += BLOCK) {
         const int32_t y = a.ind[ou + i];
         const uint32_t o = (uint32_t)(y - lo0);
         if (o < words * 32u) atomicOr(&bits[WS * (o >> 5)], 1u << (o & 31u));  // rows are sorted: always true
-        if constexpr (W) bval[i] = a.vw[y];  // the i-th key in id order has rank i
+        if constexpr (W) bval[1 + i] = (double)a.vw[y];  // the i-th key in id order has rank i
       }
-      if (W && tid == 0) bval[du] = 0.0f;
+      if (W && tid == 0) bval[0] = 0.0;
       __syncthreads();
       if constexpr (W) {
         const uint32_t per = (words + BLOCK - 1) / BLOCK;
@@ -1074,7 +1132,7 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
         for (uint32_t i = w0; i < w1; ++i) c += __popc(bits[2 * i]);
         int r = block_excl_scan(c, s_wsum);
         for (uint32_t i = w0; i < w1; ++i) {
-          bits[2 * i + 1] = (uint32_t)r;
+          bits[2 * i + 1] = (uint32_t)r + 1u;
           r += __popc(bits[2 * i]);
         }
         __syncthreads();
@@ -1093,8 +1151,7 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
       cx.pacc = pacc;
       cx.bits = bits;
       cx.bval = bval;
-      cx.span = (uint32_t)(hi0 - lo0) + 1u;
-      cx.bzero = (uint32_t)du;
+      cx.wzero = words;
       for (;;) {
         int un = 0;
         if (lane == 0) un = atomicAdd(&s_next, 1);
@@ -1419,7 +1476,12 @@ static hipError_t run_engine_t(const EngineArgs& base, void* ws, bool fixup, hip
   if (rows > 0) {
     const int g = std::min<int64_t>((rows + PB - 1) / PB, (int64_t)cus * 8);
     using Cfg = BlockCfg<W>;
-    const size_t bm_budget = Cfg::key_bytes + Cfg::val_bytes;
+    // bitmap tasks (single-chunk: no cursors or partials) take everything
+    // but the item list with one workgroup per CU, or 64 KiB with two
+    const int bm_per_cu = a.bitmap_blocks_per_cu == 2 ? 2 : 1;
+    const size_t bm_budget = bm_per_cu == 2 ? kBitmapLds2 : Cfg::lds_bytes - Cfg::item_bytes;
+    const size_t lds1 = bm_budget + Cfg::item_bytes;
+    a.bm_budget = (uint32_t)bm_budget;
     {
       ProfScope ps("sched", stream);
       hipLaunchKernelGGL((k_sched_count<W, PB>), dim3(g), dim3(PB), 0, stream, rows, a.off, a.ind, sc, a.small_max,
@@ -1433,20 +1495,22 @@ static hipError_t run_engine_t(const EngineArgs& base, void* ws, bool fixup, hip
     // hash-table tasks, then bitmap tasks: one persistent grid each (a grid
     // whose list is empty returns at once)
     int occ0 = 0, occ1 = 0;
-    for (const void* f : {reinterpret_cast<const void*>(k_owner_block<MODE, W, BB, 0>),
-                          reinterpret_cast<const void*>(k_owner_block<MODE, W, BB, 1>)}) {
-      err = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)Cfg::lds_bytes);
-      if (err != hipSuccess) return err;
-    }
+    err = hipFuncSetAttribute(reinterpret_cast<const void*>(k_owner_block<MODE, W, BB, 0>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)Cfg::lds_bytes);
+    if (err != hipSuccess) return err;
+    err = hipFuncSetAttribute(reinterpret_cast<const void*>(k_owner_block<MODE, W, BB, 1>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds1);
+    if (err != hipSuccess) return err;
     err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ0, k_owner_block<MODE, W, BB, 0>, BB, Cfg::lds_bytes);
     if (err != hipSuccess) return err;
-    err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ1, k_owner_block<MODE, W, BB, 1>, BB, Cfg::lds_bytes);
+    err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ1, k_owner_block<MODE, W, BB, 1>, BB, lds1);
     if (err != hipSuccess) return err;
     if (occ0 < 1 || occ1 < 1) return hipErrorInvalidConfiguration;
+    occ1 = std::min(occ1, bm_per_cu);
     {
       ProfScope ps("owner_block", stream);
       hipLaunchKernelGGL((k_owner_block<MODE, W, BB, 0>), dim3(cus * occ0), dim3(BB), Cfg::lds_bytes, stream, a);
-      hipLaunchKernelGGL((k_owner_block<MODE, W, BB, 1>), dim3(cus * occ1), dim3(BB), Cfg::lds_bytes, stream, a);
+      hipLaunchKernelGGL((k_owner_block<MODE, W, BB, 1>), dim3(cus * occ1), dim3(BB), lds1, stream, a);
       if ((err = hipGetLastError()) != hipSuccess) return err;
     }
     const int sg = std::min<int64_t>((rows + (SB / kWave) - 1) / (SB / kWave), (int64_t)cus * a.small_blocks_per_cu);

@@ -58,6 +58,8 @@ struct EngineArgs {
   int small_blocks_per_cu;    // grid of the one-wave-per-row kernel
   int wide_index;             // 1: rows are read with 64-bit addresses (set by the
                               // launcher when m * 4 does not fit 32-bit offsets)
+  int bitmap_blocks_per_cu;   // workgroups per CU of the bitmap-task kernel (1 or 2)
+  uint32_t bm_budget;         // bytes of the bitmap table region (set by the launcher)
   // Row segments (device memory, optional): when nseg > 0 only the rows of
   // the segments are computed. segs[i] = (first row, running row count before
   // segment i) for i < nseg, and segs[nseg] = (0, total rows); seg_rows and

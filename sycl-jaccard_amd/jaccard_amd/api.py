@@ -114,7 +114,8 @@ def _empty_f32(n, device):
 class JcTuning(ctypes.Structure):
     """``jc_tuning``: launch parameters that do not change results."""
     _fields_ = [("small_max", ctypes.c_int32), ("small_blocks_per_cu", ctypes.c_int32),
-                ("wide_index", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5)]
+                ("wide_index", ctypes.c_int32), ("bitmap_blocks_per_cu", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 4)]
 
 
 def jaccard(graph: DeviceCSR, result=None, weights=None, edge_centric: bool = True, stream=None,

@@ -10,7 +10,8 @@ order -- which the union changes -- does not matter).
 ``BatchRunner`` keeps the union resident in HBM with a preallocated
 workspace, picks launch parameters with a small autotuner (degree-bin
 threshold between the one-wave-per-row kernel and the workgroup kernel, grid
-of the former; none of them changes results), and captures the whole step in
+of the former, one or two workgroups per CU for the bitmap-table kernel; none
+of them changes results), and captures the whole step in
 a HIP graph that is replayed per batch. ``PlainRunner`` is the comparison:
 one ``jc_jaccard`` call per graph, back to back on one stream.
 
@@ -100,7 +101,8 @@ JcTuning = api.JcTuning
 def _tuned(dg, weights, out, ws, tuning, stream):
     lib = load_jaccard_lib()
     g = dg.c_struct()
-    t = None if tuning is None else ctypes.byref(JcTuning(tuning[0], tuning[1]))
+    t = None if tuning is None else ctypes.byref(
+        JcTuning(small_max=tuning[0], small_blocks_per_cu=tuning[1], bitmap_blocks_per_cu=tuning[2]))
     code = lib.jc_jaccard_tuned(ctypes.byref(g), None if weights is None else weights.data_ptr(), out.data_ptr(),
                                 1, api.FLAG_ASSUME_SYMMETRIC, ws.data_ptr(), ws.numel(), t, api._stream(stream))
     if code != 0:
@@ -110,7 +112,8 @@ def _tuned(dg, weights, out, ws, tuning, stream):
 class BatchRunner:
     """Device-resident union batches, one captured HIP graph per step."""
 
-    TUNING_SPACE = [(128, 16), (64, 16), (32, 16), (128, 32), (64, 32), (16, 16)]
+    # (small_max, small_blocks_per_cu, bitmap_blocks_per_cu)
+    TUNING_SPACE = [(128, 16, 1), (64, 16, 1), (128, 32, 1), (64, 32, 1), (128, 16, 2), (128, 32, 2)]
 
     def __init__(self, batches: List[GraphBatch], device="cuda"):
         torch = api._torch()

@@ -109,7 +109,7 @@ typedef struct jc_tuning {
                                   them); for testing both load paths */
   int32_t bitmap_blocks_per_cu; /* workgroups per CU of the bitmap-table task kernel
                                   (1 or 2, default 1). With 2 its LDS table region
-                                  shrinks to 64 KiB so two 1024-thread workgroups share
+                                  shrinks to 48 KiB so two 1024-thread workgroups share
                                   a CU; rows whose bitmap no longer fits use the hash
                                   table kernel instead */
   int32_t reserved[4];

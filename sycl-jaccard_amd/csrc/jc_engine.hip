@@ -937,7 +937,7 @@ __global__ __launch_bounds__(BLOCK) void k_owner_small(EngineArgs a) {
 #ifndef JC_BLOCK
 #define JC_BLOCK 1024
 #endif
-constexpr size_t kBlockLds = JC_BLOCK_LDS_KIB * 1024 - 512;  // minus the kernel's static shared variables
+constexpr size_t kBlockLds = JC_BLOCK_LDS_KIB * 1024 - 1024;  // minus the kernel's static shared variables
 
 #ifndef JC_BITMAP
 #define JC_BITMAP 1
@@ -971,8 +971,8 @@ __device__ __forceinline__ int block_excl_scan(int v, int* s_wsum) {
 }
 
 // Table region of the bitmap-task kernel when two workgroups share a CU:
-// 2 x (64 KiB + item list + static LDS) fits the 160 KiB of a CU.
-constexpr size_t kBitmapLds2 = 64 * 1024;
+// 2 x (48 KiB + item list + staging + static LDS) fits the 160 KiB of a CU.
+constexpr size_t kBitmapLds2 = 48 * 1024;
 
 template <bool W>
 struct BlockCfg {
@@ -986,6 +986,9 @@ struct BlockCfg {
   static constexpr size_t key_bytes = (size_t)NB * kBucket * sizeof(int32_t);
   static constexpr size_t val_bytes = W ? key_bytes : 0;
   static constexpr size_t lds_bytes = key_bytes + val_bytes + item_bytes + cursor_bytes + part_bytes;
+  // bitmap tasks: staging of the next task's entries (column, its offset and
+  // degree, and weighted, its vertex weight)
+  static constexpr size_t stage_bytes = (size_t)kTaskSpan * (3 * sizeof(int32_t) + (W ? sizeof(float) : 0));
 };
 
 template <int MODE, bool W, bool MULTI, bool BM = false>
@@ -1036,7 +1039,18 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
   unsigned char* partp = KIND == 1 ? nullptr : items + Cfg::item_bytes + Cfg::cursor_bytes;
   uint32_t* pcnt = reinterpret_cast<uint32_t*>(partp);
   DD* pacc = reinterpret_cast<DD*>(partp);
+  // Bitmap tasks look one task ahead: while the other waves run a task's
+  // units, wave 0 reserves the next task and stages its header and entries
+  // (column, offset, degree; weighted whole-row tasks also the columns'
+  // vertex weights) in LDS after the item list, so a task starts without a
+  // global round trip.
+  int32_t* st_x = KIND == 1 ? reinterpret_cast<int32_t*>(items + Cfg::item_bytes) : nullptr;
+  int32_t* st_ox = KIND == 1 ? st_x + kTaskSpan : nullptr;
+  int32_t* st_dx = KIND == 1 ? st_x + 2 * kTaskSpan : nullptr;
+  float* st_w = KIND == 1 && W ? reinterpret_cast<float*>(st_x + 3 * kTaskSpan) : nullptr;
   __shared__ int s_task;
+  __shared__ int s_pre;      // bitmap tasks: s_task's header and entries are staged
+  __shared__ int s_hdr[6];   // staged header: u, ou, du, ka, first and last column of u
   __shared__ int s_next;
   __shared__ int s_wcnt[kClasses][NW];
   __shared__ int s_cbase[kClasses + 1];   // item ranges per class, heaviest first
@@ -1048,21 +1062,40 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
   const int ntasks = a.num_tasks[KIND];
   const int tbase = KIND == 0 ? 0 : a.num_tasks[0];
   const bool fast = exact_f64(a);
+  if (KIND == 1 && tid == 0) {
+    s_task = atomicAdd(&a.task_counter[KIND], 1);
+    s_pre = 0;
+  }
   for (;;) {
     __syncthreads();
-    if (tid == 0) s_task = atomicAdd(&a.task_counter[KIND], 1);
-    __syncthreads();
+    if constexpr (KIND == 0) {
+      if (tid == 0) s_task = atomicAdd(&a.task_counter[KIND], 1);
+      __syncthreads();
+    }
     const int t = s_task;
     if (t >= ntasks) return;
-    const int2 tk = a.tasks[tbase + t];
-    const int32_t u = tk.x;
-    const int ou = a.off[u];
-    const int du = a.off[u + 1] - ou;
-    const int ka = ou + tk.y * kTaskSpan;
+    const bool pre = KIND == 1 && s_pre != 0;
+    int32_t u, lo0, hi0;
+    int ou, du, ka;
+    if (pre) {
+      u = s_hdr[0];
+      ou = s_hdr[1];
+      du = s_hdr[2];
+      ka = s_hdr[3];
+      lo0 = s_hdr[4];
+      hi0 = s_hdr[5];
+    } else {
+      const int2 tk = a.tasks[tbase + t];
+      u = tk.x;
+      ou = a.off[u];
+      du = a.off[u + 1] - ou;
+      ka = ou + tk.y * kTaskSpan;
+      lo0 = a.ind[ou];
+      hi0 = a.ind[ou + du - 1];
+    }
     const int span = min(kTaskSpan, ou + du - ka);
     // bitmap tasks: the row's id span fits the table region of the LDS
     // (bitmap_fits, decided by the scheduler)
-    const int32_t lo0 = a.ind[ou], hi0 = a.ind[ou + du - 1];
     const uint32_t words = ((uint32_t)(hi0 - lo0) >> 5) + 1u;
     const int nch = KIND == 1 ? 1 : (du + CH - 1) / CH;
 
@@ -1070,9 +1103,15 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
     int32_t x = 0;
     int ox = 0, dx = 0, cls = -1;
     if (tid < span) {
-      x = a.ind[ka + tid];
-      ox = a.off[x];
-      dx = a.off[x + 1] - ox;
+      if (pre) {
+        x = st_x[tid];
+        ox = st_ox[tid];
+        dx = st_dx[tid];
+      } else {
+        x = a.ind[ka + tid];
+        ox = a.off[x];
+        dx = a.off[x + 1] - ox;
+      }
       if (owns(u, du, x, dx)) cls = len_class((dx + nch - 1) / nch);
     }
     int rank = 0;
@@ -1117,11 +1156,20 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
       for (uint32_t i = tid; i < WS * (words + 1u); i += BLOCK) bits[i] = 0u;
       if (tid == 0) s_next = 0;
       __syncthreads();
-      for (int i = tid; i < du; i += BLOCK) {
-        const int32_t y = a.ind[ou + i];
-        const uint32_t o = (uint32_t)(y - lo0);
-        if (o < words * 32u) atomicOr(&bits[WS * (o >> 5)], 1u << (o & 31u));  // rows are sorted: always true
-        if constexpr (W) bval[1 + i] = (double)a.vw[y];  // the i-th key in id order has rank i
+      if (du <= kTaskSpan) {
+        // the task is the whole row: its entries are the keys, in id order
+        if (tid < du) {
+          const uint32_t o = (uint32_t)(x - lo0);
+          if (o < words * 32u) atomicOr(&bits[WS * (o >> 5)], 1u << (o & 31u));  // rows are sorted: always true
+          if constexpr (W) bval[1 + tid] = (double)(pre ? st_w[tid] : a.vw[x]);
+        }
+      } else {
+        for (int i = tid; i < du; i += BLOCK) {
+          const int32_t y = a.ind[ou + i];
+          const uint32_t o = (uint32_t)(y - lo0);
+          if (o < words * 32u) atomicOr(&bits[WS * (o >> 5)], 1u << (o & 31u));  // rows are sorted: always true
+          if constexpr (W) bval[1 + i] = (double)a.vw[y];  // the i-th key in id order has rank i
+        }
       }
       if (W && tid == 0) bval[0] = 0.0;
       __syncthreads();
@@ -1152,6 +1200,65 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
       cx.bits = bits;
       cx.bval = bval;
       cx.wzero = words;
+      if (w == 0) {
+        // lookahead (wave 0): reserve the next task and stage it; the loads
+        // of all its entries are issued together (three round trips)
+        int tn = 0;
+        if (lane == 0) tn = atomicAdd(&a.task_counter[KIND], 1);
+        tn = __shfl(tn, 0);
+        if (tn < ntasks) {
+          const int2 tk2 = a.tasks[tbase + tn];
+          const int ou2 = a.off[tk2.x];
+          const int du2 = a.off[tk2.x + 1] - ou2;
+          const int ka2 = ou2 + tk2.y * kTaskSpan;
+          const int span2 = min(kTaskSpan, ou2 + du2 - ka2);
+          const int32_t lo2 = a.ind[ou2], hi2 = a.ind[ou2 + du2 - 1];
+          // loads are unconditional, their index clamped to the task's last
+          // entry (span2 >= 1), so none waits behind a branch; only the
+          // stores are limited to the task
+          constexpr int NI = (kTaskSpan + kWave - 1) / kWave;
+          int32_t xs[NI];
+#pragma unroll
+          for (int k = 0; k < NI; ++k) xs[k] = a.ind[ka2 + min(lane + k * kWave, span2 - 1)];
+          int o0[NI], o1[NI];
+#pragma unroll
+          for (int k = 0; k < NI; ++k) {
+            o0[k] = a.off[xs[k]];
+            o1[k] = a.off[xs[k] + 1];
+          }
+          if constexpr (W) {
+            if (du2 <= kTaskSpan) {
+              float ws[NI];
+#pragma unroll
+              for (int k = 0; k < NI; ++k) ws[k] = a.vw[xs[k]];
+#pragma unroll
+              for (int k = 0; k < NI; ++k)
+                if (lane + k * kWave < span2) st_w[lane + k * kWave] = ws[k];
+            }
+          }
+#pragma unroll
+          for (int k = 0; k < NI; ++k) {
+            const int i = lane + k * kWave;
+            if (i < span2) {
+              st_x[i] = xs[k];
+              st_ox[i] = o0[k];
+              st_dx[i] = o1[k] - o0[k];
+            }
+          }
+          if (lane == 0) {
+            s_hdr[0] = tk2.x;
+            s_hdr[1] = ou2;
+            s_hdr[2] = du2;
+            s_hdr[3] = ka2;
+            s_hdr[4] = lo2;
+            s_hdr[5] = hi2;
+          }
+        }
+        if (lane == 0) {
+          s_task = tn;
+          s_pre = tn < ntasks ? 1 : 0;
+        }
+      }
       for (;;) {
         int un = 0;
         if (lane == 0) un = atomicAdd(&s_next, 1);
@@ -1476,11 +1583,13 @@ static hipError_t run_engine_t(const EngineArgs& base, void* ws, bool fixup, hip
   if (rows > 0) {
     const int g = std::min<int64_t>((rows + PB - 1) / PB, (int64_t)cus * 8);
     using Cfg = BlockCfg<W>;
-    // bitmap tasks (single-chunk: no cursors or partials) take everything
-    // but the item list with one workgroup per CU, or 64 KiB with two
+    // bitmap tasks (single-chunk: no cursors or partials) take all of the
+    // LDS but the item list and the staging area with one workgroup per CU,
+    // or 48 KiB with two
     const int bm_per_cu = a.bitmap_blocks_per_cu == 2 ? 2 : 1;
-    const size_t bm_budget = bm_per_cu == 2 ? kBitmapLds2 : Cfg::lds_bytes - Cfg::item_bytes;
-    const size_t lds1 = bm_budget + Cfg::item_bytes;
+    const size_t bm_budget =
+        bm_per_cu == 2 ? kBitmapLds2 : kBlockLds - Cfg::item_bytes - Cfg::stage_bytes;
+    const size_t lds1 = bm_budget + Cfg::item_bytes + Cfg::stage_bytes;
     a.bm_budget = (uint32_t)bm_budget;
     {
       ProfScope ps("sched", stream);

@@ -152,6 +152,29 @@ def run_batched(args):
             "coeffs_per_s_batched": (bu.union.m + bw.union.m) / batched_s}
 
 
+def profile_batched(args):
+    """The batched form alone: generation, autotuning and capture, then
+    args.steps timed replays of the captured step (no plain calls)."""
+    import torch
+    from jaccard_amd import batch
+    bu, bw = batch.make_batches(count=args.batch_count, seed=1)
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(0)
+    br = batch.BatchRunner([bu, bw], dev)
+    br.autotune(reps=1)
+    br.capture()
+    br.replay()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        br.replay()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t1) * 1e3 / args.steps
+    print(f"[bench] batched only: {ms:.1f} ms per replay, tuning {br.tuning}", file=sys.stderr, flush=True)
+    return {"mode": "profile_batched", "graphs": bu.count + bw.count, "stored_edges": int(bu.union.m + bw.union.m),
+            "batched_ms": ms, "replays": args.steps, "tuning": [list(t) for t in br.tuning]}
+
+
 def run_served(args):
     """Serving workload: 64 resident RMAT graphs (scales 14..20 cycling, edge
     factor 16, float32 weights in (0,1]); requests pick a graph uniformly and are
@@ -255,6 +278,9 @@ def main():
     ap.add_argument("--ingest", action="store_true", help="also time device graph ingest of the RMAT-22 draws")
     ap.add_argument("--serve-requests", type=int, default=512)
     ap.add_argument("--serve-qps", type=float, default=0.0, help="Poisson rate (0: 0.7 x plain capacity)")
+    ap.add_argument("--profile-batched", action="store_true",
+                    help="only the batched form's captured step, --steps replays after autotuning "
+                         "(for a kernel trace of it alone); prints one JSON line about it")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="also run 3 device-to-device copies of BYTES (a known-size kernel that "
                          "calibrates FETCH_SIZE/WRITE_SIZE counters)")
@@ -266,6 +292,10 @@ def main():
 
     import torch
     from jaccard_amd import api
+
+    if args.profile_batched:
+        print(json.dumps(profile_batched(args)))
+        return
 
     spec, weighted, desc = WORKLOADS[args.workload]
     t0 = time.time()

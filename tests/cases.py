@@ -61,15 +61,15 @@ def sparse_hub_graph(n=1 << 21, k=30000, extra=8, seed=4):
 
 def span_hubs_graph(k=3000, extra=6, seed=6):
     """Three hub rows of degree k whose id spans straddle the LDS bitmap
-    limits of the task kernels (128 KiB: 8 bytes per 32 ids plus 4 per key
-    weighted, 4 bytes per 32 ids unweighted): span 400,000 fits both, 600,000
-    only the unweighted bitmap, 1,100,000 neither, so one graph sends rows to
-    both task lists of either mode. Random edges among each hub's neighbours
-    give the pairs common neighbours."""
+    limits of the task kernels (about 136 KiB weighted: 8 bytes per 32 ids
+    plus 8 per key; about 139 KiB unweighted: 4 bytes per 32 ids): span
+    400,000 fits both, 600,000 only the unweighted bitmap, 1,300,000 neither,
+    so one graph sends rows to both task lists of either mode. Random edges
+    among each hub's neighbours give the pairs common neighbours."""
     rng = np.random.default_rng(seed)
-    n = 1_100_000 + 16
+    n = 1_300_000 + 16
     src, dst = [], []
-    for hub, span in ((0, 400_000), (1, 600_000), (2, 1_100_000)):
+    for hub, span in ((0, 400_000), (1, 600_000), (2, 1_300_000)):
         nb = np.unique(np.linspace(3, span + 2, k).astype(np.int64))
         src.append(np.full(nb.size, hub))
         dst.append(nb)

@@ -107,11 +107,12 @@ typedef struct jc_tuning {
                                   the index array is small enough for 32-bit buffer
                                   offsets (arrays of 2^30 entries or more always use
                                   them); for testing both load paths */
-  int32_t bitmap_blocks_per_cu; /* workgroups per CU of the bitmap-table task kernel
-                                  (1 or 2, default 1). With 2 its LDS table region
-                                  shrinks to 48 KiB so two 1024-thread workgroups share
-                                  a CU; rows whose bitmap no longer fits use the hash
-                                  table kernel instead */
+  int32_t bitmap_blocks_per_cu; /* workgroups per CU asked for the bitmap-table task
+                                  kernel (1 or 2, default 1). With 2 its LDS table
+                                  region shrinks to 48 KiB (rows whose bitmap no longer
+                                  fits use the hash-table kernel); the grid is the CU
+                                  count times what the kernel's registers and LDS let
+                                  reside, at most this value */
   int32_t reserved[4];
 } jc_tuning;
 

@@ -113,7 +113,7 @@ class BatchRunner:
     """Device-resident union batches, one captured HIP graph per step."""
 
     # (small_max, small_blocks_per_cu, bitmap_blocks_per_cu)
-    TUNING_SPACE = [(128, 16, 1), (64, 16, 1), (128, 32, 1), (64, 32, 1), (128, 16, 2), (128, 32, 2)]
+    TUNING_SPACE = [(128, 16, 1), (64, 16, 1), (128, 32, 1), (64, 32, 1), (128, 32, 2)]
 
     def __init__(self, batches: List[GraphBatch], device="cuda"):
         torch = api._torch()

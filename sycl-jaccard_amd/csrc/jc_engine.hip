@@ -9,9 +9,9 @@
 // Design for gfx950:
 //   * Each undirected pair {u, x} is computed once, by the endpoint with the
 //     larger (degree, id) -- its "owner" -- and written to both stored
-//     directions (the mirror position is found while streaming or by one
-//     search). Work is the sum over pairs of min(d_u, d_x): half of the
-//     reference's.
+//     directions (the mirror position is counted while streaming, or found
+//     by one group search for rows split over several chunks). Work is the
+//     sum over pairs of min(d_u, d_x): half of the reference's.
 //   * The owner's row is staged in LDS as a bucketized hash table: buckets
 //     of 8 slots (32 bytes as two 16-byte halves in separate arrays, two
 //     ds_read_b128), at most two keys per bucket on average weighted and
@@ -31,7 +31,8 @@
 //     counters), one queue and kernel instantiation per table kind (LDS
 //     bitmap when the row's id span fits, else hash table). A task sorts its
 //     owned entries into length classes, heaviest first, and its waves take
-//     class units from an LDS counter.
+//     class units from an LDS counter. Bitmap tasks look one task ahead: wave
+//     0 stages the next task's entries in LDS while the others run units.
 //   * Rows longer than one LDS chunk are processed chunk by chunk; each short
 //     row resumes where the previous chunk stopped (no searches), its
 //     partial result kept in LDS.

@@ -783,7 +783,8 @@ __device__ __forceinline__ void unit(const EngineArgs& a, const Chunk<W>& cx, bo
 // Length classes of a short row (entries streamed per chunk):
 //   0: <= 8     G 8,  U 1      3: <= 64   G 16, U 4
 //   1: <= 16    G 16, U 1      4: <= 256  G 32, U 4
-//   2: <= 32    G 16, U 2      5: > 256   G 64, U 4 (weighted: 3)
+//   2: <= 32    G 16, U 2      5: > 256   G 64, U 4 (weighted: 3;
+//                                          unweighted bitmap rows: 6)
 #ifndef JC_U4
 #define JC_U4 4
 #endif
@@ -792,6 +793,18 @@ __device__ __forceinline__ void unit(const EngineArgs& a, const Chunk<W>& cx, bo
 #endif
 #ifndef JC_U5W
 #define JC_U5W 3  // weighted probes hold more registers: three loads per lane measured best
+#endif
+// the same for rows probed against a bitmap table; the cheaper unweighted
+// bitmap probe keeps more loads in flight (six measured best on the batched
+// workload: 2 and 3 slower, 8 no better)
+#ifndef JC_U4BM
+#define JC_U4BM JC_U4
+#endif
+#ifndef JC_U5BM
+#define JC_U5BM 6
+#endif
+#ifndef JC_U5WBM
+#define JC_U5WBM JC_U5W
 #endif
 constexpr int kClasses = 6;
 __host__ __device__ __forceinline__ int len_class(int len) {
@@ -805,7 +818,11 @@ template <int MODE, bool W, bool MULTI, int C, bool BM = false, bool SMALL = fal
 __device__ __forceinline__ void unit_class(const EngineArgs& a, const Chunk<W>& cx, bool valid, int32_t x,
                                            int ox, int dx, int j, int item, bool fast) {
   constexpr int G = class_g(C);
-  constexpr int U = C <= 1 ? 1 : C == 2 ? 2 : C == 3 ? 4 : C == 4 ? JC_U4 : (W ? JC_U5W : JC_U5);
+  constexpr int U = C <= 1 ? 1
+                  : C == 2 ? 2
+                  : C == 3 ? 4
+                  : C == 4 ? (BM ? JC_U4BM : JC_U4)
+                           : (W ? (BM ? JC_U5WBM : JC_U5W) : (BM ? JC_U5BM : JC_U5));
   // classes 0-3 stream a row (at most 64 entries) in one pass of G * U; the
   // single-pass mirror count and its byte-packed reduction rely on it
   static_assert(C > 3 ? G * U > 64 : G * U >= (C == 0 ? 8 : C == 1 ? 16 : C == 2 ? 32 : 64), "class shape");

@@ -108,6 +108,31 @@ def cpu_baseline(g, budget_s=15.0, seed=0):
                       f"binary-search loop (jaccard.cpp:711-742) restated in C/OpenMP, {t_used:.1f} s"}
 
 
+def oracle_check(g, out, W, weighted, n_sample=200000, seed=3):
+    """Part of the CPU-baseline leg: the oracle, as the checker, on a random
+    sample of stored edges of the measured step's device output. Weighted:
+    the FP64 quotient (worst relative error reported against the 1e-6
+    tolerance); unweighted: the reference loop's fp32 result, bit for bit."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    rng = np.random.default_rng(seed)
+    k = np.arange(g.m) if n_sample >= g.m else np.unique(rng.integers(0, g.m, n_sample))
+    got = out.cpu().numpy()[k]
+    if weighted:
+        rows = (np.searchsorted(g.offsets, k, side="right") - 1).astype(np.int32)
+        cols = g.indices[k].astype(np.int32)
+        ref = oracle.pairs_weighted_f64(g, W.cpu().numpy().astype(np.float64), rows, cols)
+        nz = ref != 0
+        rel = np.abs(got[nz].astype(np.float64) - ref[nz]) / np.abs(ref[nz])
+        zero_ok = bool(np.all(got[~nz] == 0))
+        return {"sample_edges": int(k.size), "max_rel_err": float(rel.max()) if rel.size else 0.0,
+                "tolerance": 1e-6, "zeros_exact": zero_ok, "reference": "FP64 oracle, per-pair weighted quotient"}
+    ref = oracle.ec_edges(g, k)
+    return {"sample_edges": int(k.size), "bit_exact": bool(np.array_equal(got.view(np.uint32), ref.view(np.uint32))),
+            "reference": "oracle restatement of the reference edge-centric loop (jaccard.cpp:711-742)"}
+
+
 def run_batched(args):
     """Batched workload: 4096 RMAT graphs (n log-uniform in [2^8, 2^16], edge
     factor uniform in [4, 32], half weighted) in one captured HIP graph vs one
@@ -394,6 +419,8 @@ def main():
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline...", file=sys.stderr, flush=True)
         cpu = cpu_baseline(g, args.cpu_budget)
+        extras["oracle_check"] = oracle_check(g, out, W if weighted else None, weighted)
+        print(f"[bench] oracle check: {extras['oracle_check']}", file=sys.stderr, flush=True)
 
     line = {
         "metric": METRIC, "value": value, "unit": "coeffs/s", "n_gpus": 1, "steps": args.steps,

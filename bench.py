@@ -81,31 +81,49 @@ def algorithmic_bytes(g, st, weighted):
     return int(4 * st["sum_min_deg"]) + compulsory_bytes(g, weighted)
 
 
-def cpu_baseline(g, budget_s=15.0, seed=0):
-    """Reference EC loop (oracle restatement) on a random edge sample, all host threads."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(g, budget_s=15.0, seed=0, runs=3):
+    """Reference EC loop (oracle restatement) on a random edge sample, all host
+    threads: a sample sized to about budget_s / runs seconds, timed `runs`
+    times, median rate."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     threads = oracle.max_threads()
     rng = np.random.default_rng(seed)
-    n_s = 2000
-    t_used = 0.0
-    done = 0
+    # size the sample: double until one pass takes a tenth of a run's share
+    n_s, dt = 2000, 0.0
     while True:
         edges = rng.integers(0, g.m, n_s)
         t0 = time.perf_counter()
         oracle.ec_edges(g, edges, threads)
         dt = time.perf_counter() - t0
-        t_used += dt
-        done += n_s
-        if t_used >= budget_s or dt <= 0:
+        if dt >= 0.1 * budget_s / runs or n_s >= g.m:
             break
-        n_s = int(min(max(n_s * 2, 1000), n_s * (budget_s - t_used) / dt + 1))
-        if n_s < 1000:
-            break
-    return {"value": done / t_used, "unit": "coeffs/s", "cores": threads, "kind": "port",
-            "sample": f"{done} uniformly sampled stored edges of the same graph, reference edge-centric "
-                      f"binary-search loop (jaccard.cpp:711-742) restated in C/OpenMP, {t_used:.1f} s"}
+        n_s *= 2
+    n_s = int(min(g.m, max(n_s, n_s * (budget_s / runs) / max(dt, 1e-9))))
+    edges = rng.integers(0, g.m, n_s)
+    rates = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        oracle.ec_edges(g, edges, threads)
+        rates.append(n_s / (time.perf_counter() - t0))
+    rates.sort()
+    med = rates[len(rates) // 2]
+    return {"value": med, "unit": "coeffs/s", "cores": threads, "kind": "port", "cpu": cpu_model(),
+            "runs": [round(r, 1) for r in rates],
+            "sample": f"{n_s} uniformly sampled stored edges of the same graph, reference edge-centric "
+                      f"binary-search loop (jaccard.cpp:711-742) restated in C/OpenMP, median of {runs} runs "
+                      f"(about {n_s / med:.1f} s each)"}
 
 
 def oracle_check(g, out, W, weighted, n_sample=200000, seed=3):
@@ -167,8 +185,8 @@ def run_batched(args):
     torch.cuda.synchronize()
     batched_s = (time.perf_counter() - t1) / args.batch_reps
     print(f"[bench] batched {batched_s * 1e3:.1f} ms per batch, tuning {br.tuning}", file=sys.stderr, flush=True)
-    same = all(np.array_equal(br.member_output(k, i).cpu().numpy().view(np.uint32),
-                              out.cpu().numpy().view(np.uint32)) for k, i, _, out, _ in pr.graphs)
+    same = all(torch.equal(br.member_output(k, i).view(torch.int32), out.view(torch.int32))
+               for k, i, _, out, _ in pr.graphs)
     ngraphs = bu.count + bw.count
     return {"graphs": ngraphs, "stored_edges": int(bu.union.m + bw.union.m),
             "batched_graphs_per_s": ngraphs / batched_s, "plain_graphs_per_s": ngraphs / plain_s,
@@ -223,15 +241,16 @@ def run_served(args):
            "hbm_budget_bytes": 1 << 30, "slots": 4}
 
     def both(reqs, tag):
-        res_c = serve.result_buffers(gs, reqs, dev, share_full=True)
-        res_f = serve.result_buffers(gs, reqs, dev, share_full=True)
+        # one result buffer per request, so the identity check covers every request
+        res_c = serve.result_buffers(gs, reqs, dev)
+        res_f = serve.result_buffers(gs, reqs, dev)
         cont.run(reqs[: min(8, len(reqs))], res_c[: min(8, len(reqs))])  # warm
         st_c, info_c = cont.run(reqs, res_c)
         st_f, info_f = fifo.run(reqs, res_f)
         assert (st_c["status"] == 0).all() and (st_f["status"] == 0).all()
         torch.cuda.synchronize()
-        same = all(np.array_equal(a.cpu().numpy().view(np.uint32), b.cpu().numpy().view(np.uint32))
-                   for a, b in zip(res_c, res_f))
+        same = all(torch.equal(a.view(torch.int32), b.view(torch.int32)) for a, b in zip(res_c, res_f))
+        del res_c, res_f
         c, f = serve.latency_summary(reqs, st_c), serve.latency_summary(reqs, st_f)
         c.update(batches=info_c["batches"], max_in_flight=info_c["max_in_flight"],
                  peak_budget_bytes=info_c["peak_budget_bytes"])
@@ -243,10 +262,15 @@ def run_served(args):
 
     burst = serve.request_stream(gs, args.serve_requests, qps=0.0, seed=11)
     out["burst"] = both(burst, "burst")
-    qps = args.serve_qps or 0.7 * out["burst"]["plain"]["requests_per_s"]
-    stream = serve.request_stream(gs, args.serve_requests, qps=qps, seed=12)
-    out["poisson"] = both(stream, f"poisson {qps:.0f} qps")
-    out["poisson"]["qps"] = qps
+    plain_cap = out["burst"]["plain"]["requests_per_s"]
+    # below the plain path's capacity (latency at moderate load), then above it
+    # (the plain queue grows without bound; continuous batching absorbs it)
+    for key, qps, seed in (("poisson", args.serve_qps or 0.7 * plain_cap, 12),
+                           ("poisson_overload", 1.15 * plain_cap, 13)):
+        stream = serve.request_stream(gs, args.serve_requests, qps=qps, seed=seed)
+        out[key] = both(stream, f"{key} {qps:.0f} qps")
+        out[key]["qps"] = qps
+        out[key]["qps_over_plain_capacity"] = qps / plain_cap
     cont.close()
     fifo.close()
     return out
@@ -295,11 +319,18 @@ def main():
     ap.add_argument("--algo", default="ec", choices=["ec", "vc"])
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--extras", action="store_true", help="also time VC and unweighted EC")
-    ap.add_argument("--batched", action="store_true", help="also run the batched workload (4096 graphs)")
+    ap.add_argument("--extras", action=argparse.BooleanOptionalAction, default=True,
+                    help="also time VC (the vertex-centric pipeline) and unweighted EC (default on)")
+    ap.add_argument("--batched", action=argparse.BooleanOptionalAction, default=True,
+                    help="also run the batched workload, 4096 graphs (default on)")
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--batch-reps", type=int, default=3)
-    ap.add_argument("--served", action="store_true", help="also run the serving workload")
+    ap.add_argument("--served", action=argparse.BooleanOptionalAction, default=True,
+                    help="also run the serving workload, 64 graphs (default on)")
+    ap.add_argument("--hash-table32", action="store_true",
+                    help="32-bit hash tables for every hash-table task (jc_tuning.hash_table = 1)")
+    ap.add_argument("--core-only", action="store_true",
+                    help="only the headline step (no extras, batched or served; for profiling)")
     ap.add_argument("--ingest", action="store_true", help="also time device graph ingest of the RMAT-22 draws")
     ap.add_argument("--serve-requests", type=int, default=512)
     ap.add_argument("--serve-qps", type=float, default=0.0, help="Poisson rate (0: 0.7 x plain capacity)")
@@ -310,6 +341,8 @@ def main():
                     help="also run 3 device-to-device copies of BYTES (a known-size kernel that "
                          "calibrates FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.core_only:
+        args.extras = args.batched = args.served = False
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
@@ -336,12 +369,14 @@ def main():
     ec = args.algo == "ec"
     stream = torch.cuda.current_stream()
 
+    tuning = {"hash_table": 1} if args.hash_table32 else None
+
     def step():
         if weighted:
             api.vertex_weights(dg, out=W, stream=stream)
-            api.jaccard(dg, out, weights=W, edge_centric=ec, stream=stream)
+            api.jaccard(dg, out, weights=W, edge_centric=ec, stream=stream, tuning=tuning)
         else:
-            api.jaccard(dg, out, edge_centric=ec, stream=stream)
+            api.jaccard(dg, out, edge_centric=ec, stream=stream, tuning=tuning)
 
     for _ in range(args.warmup):
         step()
@@ -371,7 +406,7 @@ def main():
     calls = []
     for _ in range(3):
         t1 = time.perf_counter()
-        api.jaccard(dg, out, weights=W if weighted else None, edge_centric=ec, stream=stream)
+        api.jaccard(dg, out, weights=W if weighted else None, edge_centric=ec, stream=stream, tuning=tuning)
         torch.cuda.synchronize()
         calls.append((time.perf_counter() - t1) * 1e3)
     traffic = measured_traffic(args.workload)
@@ -427,7 +462,8 @@ def main():
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64" if weighted else "int32", "data": "synthetic",
         "config": {"workload": args.workload, "description": desc, "algorithm": "edge_centric" if ec else "vertex_centric",
-                   "n": g.n, "m": g.m, "weighted": weighted, **{k: v for k, v in spec.items() if k != "kind"}},
+                   "n": g.n, "m": g.m, "weighted": weighted, **{k: v for k, v in spec.items() if k != "kind"},
+                   "hash_tables": "32-bit" if args.hash_table32 else "quotient below 2^24 vertices"},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": measured_traffic(args.workload), "kernel": "k_owner_block",
                      "algorithmic_bytes": B},

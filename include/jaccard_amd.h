@@ -27,8 +27,9 @@
  *   - Unweighted results are bit-identical to the reference arithmetic
  *     I / ((float)(d_u + d_v) - I). Weighted results (vertex weights W):
  *     J = I_W / (S(u) + S(v) - I_W), S(u) = sum_{k in N(u)} W[k],
- *     I_W = sum_{k in N(u) & N(v)} W[k], accumulated exactly (double-double)
- *     and divided in fp64, rounded once to fp32.
+ *     I_W = sum_{k in N(u) & N(v)} W[k], accumulated exactly (double-double,
+ *     or plain fp64 where that is exact, jc_weighted_sum_mode) and divided in
+ *     fp64, rounded once to fp32.
  */
 #ifndef JACCARD_AMD_H
 #define JACCARD_AMD_H
@@ -113,7 +114,15 @@ typedef struct jc_tuning {
                                   fits use the hash-table kernel); the grid is the CU
                                   count times what the kernel's registers and LDS let
                                   reside, at most this value */
-  int32_t reserved[4];
+  int32_t hash_table;          /* hash-table tasks: 0 (default) quotient tables (16-bit
+                                  keys) when n < 2^24, else 32-bit tables; 1 always
+                                  32-bit tables */
+  int32_t quot_max_disp;       /* for tests: 0 (default) or k >= 1, which caps how far
+                                  (k - 1 buckets) a quotient-table key may move from its
+                                  home bucket; a task whose table then does not build is
+                                  redone with the 32-bit table (1 forces that for every
+                                  chunk with a full bucket) */
+  int32_t reserved[2];
 } jc_tuning;
 
 /* jc_jaccard_ex with launch tuning (NULL = defaults). */
@@ -220,6 +229,18 @@ int jc_validate_csr(const jc_csr *g, int *what, jc_stream_t stream);
 /* Synchronous symmetry check: *symmetric = 1 if every stored (u,v) has (v,u). */
 int jc_csr_is_symmetric(const jc_csr *g, int *symmetric, jc_stream_t stream);
 
+/* How the weighted entry points (jc_jaccard, jc_jaccard_list, the batched and
+ * served forms) add vertex weights W on graph g: JC_SUM_FP64 when every W is
+ * a non-negative multiple of 2^e and every row sum is below 2^(e + 53), so
+ * plain fp64 sums are exact; else JC_SUM_DOUBLE_DOUBLE (exact for spans up
+ * to 104 bits). Both give the same bits wherever both are exact. The decision
+ * is the one those calls make on the device for the same g and W.
+ * Synchronous. No reference counterpart (the reference adds in fp32,
+ * jaccard.cpp:53-137, 370-396). */
+#define JC_SUM_FP64 0
+#define JC_SUM_DOUBLE_DOUBLE 1
+int jc_weighted_sum_mode(const jc_csr *g, const float *vertex_weights, int *mode, jc_stream_t stream);
+
 /* Per-kernel device timing, the counterpart of the reference's EVENT_PROFILE
  * build (wait_and_print, standalone_algorithms.hpp:37-47, used at
  * jaccard.cpp:798-809, 875-896): while enabled, each kernel launch of the
@@ -268,7 +289,9 @@ typedef struct jc_server jc_server;
  * pointers, local ids; edge_w unused) and their vertex weights (device
  * arrays of n floats, or vertex_weights = NULL for unweighted Jaccard), and
  * allocates the HBM budget. Synchronous. The graphs must stay allocated for
- * the server's lifetime (JC_SERVE_FIFO reads them). */
+ * the server's lifetime (JC_SERVE_FIFO reads them). A graph that is not
+ * symmetric (some stored (u,v) without (v,u)) gives JC_ERR_INVALID_CSR:
+ * batched passes compute each pair once for both stored directions. */
 int jc_server_create(int32_t num_graphs, const jc_csr *graphs, const float *const *vertex_weights,
                      const jc_server_config *cfg, jc_server **out);
 int jc_server_destroy(jc_server *s);

@@ -32,7 +32,7 @@ class jaccard_error : public std::runtime_error {
 };
 
 // Non-owning view of a device-resident CSR graph (GraphCSRView,
-// standalone_csr.hpp:181-242): offsets[n+1], indices[m], optional edge_data[m].
+// standalone_csr.hpp:44-105): offsets[n+1], indices[m], optional edge_data[m].
 template <typename vertex_t, typename edge_t, typename weight_t>
 struct GraphCSRView {
   using vertex_type = vertex_t;

@@ -6,6 +6,7 @@
 // Temporaries come from a caller workspace or stream-ordered allocations.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
 #include <algorithm>
 
 #include "../../include/jaccard_amd.h"
@@ -96,6 +97,8 @@ int run_jaccard(const jc_csr* g, const float* vw, float* result, bool ec, unsign
   a.small_blocks_per_cu = tuning ? tuning->small_blocks_per_cu : 0;
   a.wide_index = tuning ? (tuning->wide_index != 0) : 0;
   a.bitmap_blocks_per_cu = tuning ? tuning->bitmap_blocks_per_cu : 0;
+  a.hash_table = tuning ? tuning->hash_table : 0;
+  a.quot_max_disp = tuning ? tuning->quot_max_disp : 0;
   double* S64 = weighted ? reinterpret_cast<double*>(ws + L.s64) : nullptr;
   if (weighted) {
     jc::ExactInfo* ex = reinterpret_cast<jc::ExactInfo*>(ws + L.exact);
@@ -299,6 +302,29 @@ int jc_ec_unweighted(int32_t m, int32_t n, const int32_t* offsets, const int32_t
   (void)dest_ind;
   jc_csr g{n, m, offsets, indices, nullptr};
   return jc_jaccard_ex(&g, nullptr, weight_j, JC_EDGE_CENTRIC, 0u, nullptr, 0, stream);
+}
+
+int jc_weighted_sum_mode(const jc_csr* g, const float* vertex_weights, int* mode, jc_stream_t stream) {
+  if (!csr_args_ok(g) || !vertex_weights || !mode) return JC_ERR_INVALID_ARG;
+  *mode = JC_SUM_FP64;
+  if (g->n == 0) return JC_OK;
+  hipStream_t s = hs(stream);
+  // the row sums and range facts exactly as run_jaccard computes them
+  const size_t rows_b = al(jc::rowsum_scratch_bytes(g->n));
+  const size_t s64_b = al((size_t)g->n * sizeof(double));
+  AsyncBuf buf;
+  buf.s = s;
+  JC_TRY(hipMallocAsync(&buf.p, rows_b + s64_b + al(sizeof(jc::ExactInfo)), s));
+  unsigned char* p = static_cast<unsigned char*>(buf.p);
+  double* S64 = reinterpret_cast<double*>(p + rows_b);
+  jc::ExactInfo* ex = reinterpret_cast<jc::ExactInfo*>(p + rows_b + s64_b);
+  JC_TRY(jc::launch_rowsum_f64(g->n, g->offsets, g->indices, vertex_weights, S64, p, s));
+  JC_TRY(jc::launch_exact_info(g->n, vertex_weights, S64, ex, s));
+  jc::ExactInfo h{};
+  JC_TRY(hipMemcpyAsync(&h, ex, sizeof(h), hipMemcpyDeviceToHost, s));
+  JC_TRY(hipStreamSynchronize(s));
+  *mode = jc::exact_sums_fit_f64(h) ? JC_SUM_FP64 : JC_SUM_DOUBLE_DOUBLE;
+  return JC_OK;
 }
 
 int jc_validate_csr(const jc_csr* g, int* what, jc_stream_t stream) {

@@ -1,0 +1,6 @@
+// Owner-engine kernels and launch sequence for one result mode (see jc_engine.hpp).
+#include "jc_engine.hpp"
+
+namespace jc {
+template hipError_t run_engine_t<kModeEcU, false>(const EngineArgs&, void*, bool, hipStream_t);
+}  // namespace jc

@@ -34,6 +34,20 @@ struct ExactInfo {
   int pad;
 };
 
+// True when every weight sum of a pass is exact in plain fp64: all vertex
+// weights non-negative multiples of 2^min_lsb and every row sum below
+// 2^max_exp with max_exp - min_lsb <= 52, so every partial sum of a subset of
+// a row fits 53 bits; double-double and plain fp64 then give the same bits.
+// No non-zero weight (min_lsb unset) or no non-zero row sum (max_exp unset,
+// so every weight that occurs in a row is 0) makes every sum 0: exact too.
+// The difference is taken in 64 bits (the unset values are INT32_MAX and
+// INT32_MIN).
+__host__ __device__ inline bool exact_sums_fit_f64(const ExactInfo& e) {
+  if (e.bad != 0) return false;
+  if (e.min_lsb == INT32_MAX || e.max_exp == INT32_MIN) return true;
+  return (int64_t)e.max_exp - (int64_t)e.min_lsb <= 52;
+}
+
 struct EngineArgs {
   int32_t n;
   int64_t m;
@@ -59,6 +73,8 @@ struct EngineArgs {
   int wide_index;             // 1: rows are read with 64-bit addresses (set by the
                               // launcher when m * 4 does not fit 32-bit offsets)
   int bitmap_blocks_per_cu;   // workgroups per CU of the bitmap-task kernel (1 or 2)
+  int hash_table;             // 1: 32-bit tables even where quotient tables apply
+  int quot_max_disp;          // > 0: quotient keys move at most quot_max_disp - 1 buckets
   uint32_t bm_budget;         // bytes of the bitmap table region (set by the launcher)
   // Row segments (device memory, optional): when nseg > 0 only the rows of
   // the segments are computed. segs[i] = (first row, running row count before
@@ -71,6 +87,11 @@ struct EngineArgs {
   // Weighted modes: optional range facts of vw and S64 (device memory); when
   // they show all sums are exact in fp64 the kernels add in plain fp64.
   const ExactInfo* exact;
+  // Hash-table tasks the quotient-table kernel had to abandon (device memory,
+  // set by the launcher), and whether a launch is the 32-bit redo of them.
+  int* redo;
+  int* num_redo;
+  int redo_pass;
 };
 
 // Workspace needed by an owner-engine pass over a graph with n vertices and m

@@ -22,6 +22,8 @@
 #include "jc_internal.hpp"
 
 namespace jc {
+hipError_t launch_symcheck(int32_t n, int64_t m, const int32_t* off, const int32_t* ind, int* flag,
+                           hipStream_t s);
 namespace {
 
 // Arena builders: member graph g lands at rows [vbase, vbase + n), stored
@@ -342,6 +344,25 @@ int jc_server_create(int32_t num_graphs, const jc_csr* graphs, const float* cons
       hipLaunchKernelGGL(jc::k_arena_indices, dim3(gb), dim3(256), 0, st, m.m, m.local.indices, m.vbase,
                          s->ind + m.ebase);
       if ((e = hipGetLastError()) != hipSuccess) return fail(e);
+    }
+  }
+  // The batched engine pass runs without the fix-up of unmatched positions,
+  // so every member must be symmetric: the arena (their disjoint union) is
+  // checked once here.
+  if (M > 0) {
+    int* flag = nullptr;
+    if ((e = hipMalloc(&flag, sizeof(int))) != hipSuccess) return fail(e);
+    int h = 0;
+    e = jc::launch_fill_u32(reinterpret_cast<uint32_t*>(flag), 1, 0u, st);
+    if (e == hipSuccess) e = jc::launch_symcheck(s->N, s->M, s->off, s->ind, flag, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    const hipError_t ef = hipFree(flag);
+    if (e == hipSuccess) e = ef;
+    if (e != hipSuccess) return fail(e);
+    if (h != 0) {
+      free_server(s);
+      return JC_ERR_INVALID_CSR;
     }
   }
   if (s->weighted && N > 0 &&

@@ -111,6 +111,7 @@ def _declare_jaccard(lib: ctypes.CDLL) -> None:
         "jc_ec_unweighted": [i32, i32, vp, vp, vp, vp, vp],
         "jc_validate_csr": [vp, ctypes.POINTER(I), vp],
         "jc_csr_is_symmetric": [vp, ctypes.POINTER(I), vp],
+        "jc_weighted_sum_mode": [vp, vp, ctypes.POINTER(I), vp],
         "jc_csr_from_edges": [i32, i64, vp, vp, vp, I, vp, vp, vp, i64, ctypes.POINTER(i64), vp],
         "jc_profile_enable": [I],
         "jc_profile_reset": [],

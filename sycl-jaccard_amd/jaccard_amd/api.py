@@ -68,6 +68,30 @@ def _check(code: int, what: str) -> None:
         raise JaccardError(code, what)
 
 
+def _require(t, dtype, numel: int, name: str, device=None, optional: bool = False) -> None:
+    """Raise ValueError unless ``t`` is a contiguous device tensor of ``dtype``
+    with at least ``numel`` elements (on ``device`` when given). The library
+    reads and writes through raw pointers, so a wrong dtype, a short buffer or
+    host memory must be refused here, before any device access."""
+    torch = _torch()
+    if t is None:
+        if optional:
+            return
+        raise ValueError(f"{name} is required")
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"{name} must be a torch tensor, got {type(t).__name__}")
+    if t.dtype != dtype:
+        raise ValueError(f"{name} must be {dtype}, got {t.dtype}")
+    if t.numel() < numel:
+        raise ValueError(f"{name} needs at least {numel} elements, got {t.numel()}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    if not t.is_cuda:
+        raise ValueError(f"{name} must be a device (HIP) tensor, got one on {t.device}")
+    if device is not None and t.device != device:
+        raise ValueError(f"{name} is on {t.device}, the graph on {device}")
+
+
 class DeviceCSR:
     """``GraphCSRView<int32_t,int32_t,float>`` resident on the GPU.
 
@@ -115,7 +139,8 @@ class JcTuning(ctypes.Structure):
     """``jc_tuning``: launch parameters that do not change results."""
     _fields_ = [("small_max", ctypes.c_int32), ("small_blocks_per_cu", ctypes.c_int32),
                 ("wide_index", ctypes.c_int32), ("bitmap_blocks_per_cu", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 4)]
+                ("hash_table", ctypes.c_int32), ("quot_max_disp", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 2)]
 
 
 def jaccard(graph: DeviceCSR, result=None, weights=None, edge_centric: bool = True, stream=None,
@@ -126,9 +151,15 @@ def jaccard(graph: DeviceCSR, result=None, weights=None, edge_centric: bool = Tr
     ``result``: float32[m] output (allocated when None). Returns ``result``.
     ``tuning``: dict of ``jc_tuning`` fields (``jc_jaccard_tuned``), or None.
     """
+    torch = _torch()
     lib = load_jaccard_lib()
+    dev = graph.offsets.device
     if result is None:
-        result = _empty_f32(graph.m, graph.offsets.device)
+        result = _empty_f32(graph.m, dev)
+    _require(weights, torch.float32, graph.n, "weights", dev, optional=True)
+    _require(result, torch.float32, graph.m, "result", dev)
+    if workspace is not None:
+        _require(workspace, workspace.dtype, 0, "workspace", dev)
     g = graph.c_struct()
     ws_ptr, ws_bytes = (None, 0) if workspace is None else (workspace.data_ptr(), workspace.numel() * workspace.element_size())
     algo = EDGE_CENTRIC if edge_centric else VERTEX_CENTRIC
@@ -149,12 +180,18 @@ def workspace_size(n: int, m: int, weighted: bool, edge_centric: bool) -> int:
 
 def jaccard_list(graph: DeviceCSR, first, second, result=None, weights=None, stream=None):
     """Jaccard coefficient of arbitrary vertex pairs (``sygraph::jaccard_list``)."""
+    torch = _torch()
     lib = load_jaccard_lib()
-    k = int(first.numel())
-    if second.numel() != k:
+    dev = graph.offsets.device
+    if isinstance(first, torch.Tensor) and isinstance(second, torch.Tensor) and first.numel() != second.numel():
         raise ValueError("first and second must have the same length")
+    _require(first, torch.int32, 0, "first", dev)
+    k = int(first.numel())
+    _require(second, torch.int32, k, "second", dev)
     if result is None:
-        result = _empty_f32(k, graph.offsets.device)
+        result = _empty_f32(k, dev)
+    _require(result, torch.float32, k, "result", dev)
+    _require(weights, torch.float32, graph.n, "weights", dev, optional=True)
     g = graph.c_struct()
     _check(lib.jc_jaccard_list(ctypes.byref(g), _ptr(weights), k, _ptr(first), _ptr(second), _ptr(result),
                                _stream(stream)), "jc_jaccard_list")
@@ -166,6 +203,7 @@ def vertex_weights(graph: DeviceCSR, out=None, stream=None):
     lib = load_jaccard_lib()
     if out is None:
         out = _empty_f32(graph.n, graph.offsets.device)
+    _require(out, _torch().float32, graph.n, "out", graph.offsets.device)
     g = graph.c_struct()
     _check(lib.jc_vertex_weights(ctypes.byref(g), _ptr(out), _stream(stream)), "jc_vertex_weights")
     return out
@@ -174,6 +212,8 @@ def vertex_weights(graph: DeviceCSR, out=None, stream=None):
 def vertex_weights_ex(graph: DeviceCSR, out, workspace, stream=None):
     """vertex_weights with a caller workspace (allocates nothing)."""
     lib = load_jaccard_lib()
+    _require(out, _torch().float32, graph.n, "out", graph.offsets.device)
+    _require(workspace, workspace.dtype, 0, "workspace", graph.offsets.device)
     g = graph.c_struct()
     _check(lib.jc_vertex_weights_ex(ctypes.byref(g), _ptr(out), workspace.data_ptr(),
                                     workspace.numel() * workspace.element_size(), _stream(stream)),
@@ -188,6 +228,7 @@ def vertex_weights_workspace_size(n: int) -> int:
 def nonzero_count(result, thresh: float = 1e-5, stream=None) -> int:
     """Entries above ``thresh`` (the EC path's "non zero pairs" line)."""
     lib = load_jaccard_lib()
+    _require(result, _torch().float32, 0, "result")
     c = ctypes.c_int64(0)
     _check(lib.jc_nonzero_count(int(result.numel()), _ptr(result), float(thresh), ctypes.byref(c),
                                 _stream(stream)), "jc_nonzero_count")
@@ -197,23 +238,40 @@ def nonzero_count(result, thresh: float = 1e-5, stream=None) -> int:
 # ---- kernel level ----------------------------------------------------------
 
 def fill(x, value: float, stream=None):
+    _require(x, _torch().float32, 0, "x")
     _check(load_jaccard_lib().jc_fill(int(x.numel()), _ptr(x), float(value), _stream(stream)), "jc_fill")
     return x
 
 
 def rowsum(graph: DeviceCSR, work, v=None, stream=None):
+    f32, dev = _torch().float32, graph.offsets.device
+    _require(work, f32, graph.n, "work", dev)
+    _require(v, f32, graph.n, "v", dev, optional=True)
     _check(load_jaccard_lib().jc_rowsum(graph.n, graph.m, _ptr(graph.offsets), _ptr(graph.indices), _ptr(v),
                                         _ptr(work), _stream(stream)), "jc_rowsum")
     return work
 
 
 def intersect(graph: DeviceCSR, work, weight_i, weight_s, v=None, stream=None):
+    f32, dev = _torch().float32, graph.offsets.device
+    _require(work, f32, graph.n, "work", dev)
+    _require(weight_i, f32, graph.m, "weight_i", dev)
+    _require(weight_s, f32, graph.m, "weight_s", dev)
+    _require(v, f32, graph.n, "v", dev, optional=True)
     _check(load_jaccard_lib().jc_intersect(graph.n, graph.m, _ptr(graph.offsets), _ptr(graph.indices), _ptr(v),
                                            _ptr(work), _ptr(weight_i), _ptr(weight_s), _stream(stream)),
            "jc_intersect")
 
 
 def intersect_pairs(graph: DeviceCSR, first, second, work, weight_i, weight_s, v=None, stream=None):
+    torch, dev = _torch(), graph.offsets.device
+    _require(first, torch.int32, 0, "first", dev)
+    k = int(first.numel())
+    _require(second, torch.int32, k, "second", dev)
+    _require(work, torch.float32, graph.n, "work", dev)
+    _require(weight_i, torch.float32, k, "weight_i", dev)
+    _require(weight_s, torch.float32, k, "weight_s", dev)
+    _require(v, torch.float32, graph.n, "v", dev, optional=True)
     _check(load_jaccard_lib().jc_intersect_pairs(graph.n, int(first.numel()), _ptr(graph.offsets),
                                                  _ptr(graph.indices), _ptr(first), _ptr(second), _ptr(v),
                                                  _ptr(work), _ptr(weight_i), _ptr(weight_s), _stream(stream)),
@@ -221,17 +279,27 @@ def intersect_pairs(graph: DeviceCSR, first, second, work, weight_i, weight_s, v
 
 
 def jw(weight_i, weight_s, weight_j, stream=None):
+    f32 = _torch().float32
+    _require(weight_i, f32, 0, "weight_i")
+    _require(weight_s, f32, weight_i.numel(), "weight_s", weight_i.device)
+    _require(weight_j, f32, weight_i.numel(), "weight_j", weight_i.device)
     _check(load_jaccard_lib().jc_jw(int(weight_i.numel()), _ptr(weight_i), _ptr(weight_s), _ptr(weight_j),
                                     _stream(stream)), "jc_jw")
     return weight_j
 
 
 def ec_scan(graph: DeviceCSR, dest_ind, weight_j, stream=None):
+    torch, dev = _torch(), graph.offsets.device
+    _require(dest_ind, torch.int32, graph.m, "dest_ind", dev)
+    _require(weight_j, torch.float32, graph.m, "weight_j", dev)
     _check(load_jaccard_lib().jc_ec_scan(graph.m, graph.n, _ptr(graph.offsets), _ptr(dest_ind), _ptr(weight_j),
                                          _stream(stream)), "jc_ec_scan")
 
 
 def ec_unweighted(graph: DeviceCSR, dest_ind, weight_j, stream=None):
+    torch, dev = _torch(), graph.offsets.device
+    _require(dest_ind, torch.int32, graph.m, "dest_ind", dev, optional=True)
+    _require(weight_j, torch.float32, graph.m, "weight_j", dev)
     _check(load_jaccard_lib().jc_ec_unweighted(graph.m, graph.n, _ptr(graph.offsets), _ptr(graph.indices),
                                                _ptr(dest_ind), _ptr(weight_j), _stream(stream)),
            "jc_ec_unweighted")
@@ -258,8 +326,11 @@ def csr_from_edges(n: int, src, dst, edge_weights=None, symmetrize: bool = True,
     weighted_out)."""
     torch = _torch()
     lib = load_jaccard_lib()
+    _require(src, torch.int32, 0, "src")
     e = int(src.numel())
     dev = src.device
+    _require(dst, torch.int32, e, "dst", dev)
+    _require(edge_weights, torch.float32, e, "edge_weights", dev, optional=True)
     cap = max(e * (2 if symmetrize else 1), 1)
     off = torch.empty(n + 1, dtype=torch.int32, device=dev)
     ind = torch.empty(cap, dtype=torch.int32, device=dev)
@@ -272,6 +343,21 @@ def csr_from_edges(n: int, src, dst, edge_weights=None, symmetrize: bool = True,
                                  ctypes.byref(m), _stream(stream)), "jc_csr_from_edges")
     mm = int(m.value)
     return DeviceCSR(n, off, ind[:mm], None if w is None else w[:mm])
+
+
+SUM_FP64 = 0
+SUM_DOUBLE_DOUBLE = 1
+
+
+def weighted_sum_mode(graph: DeviceCSR, weights, stream=None) -> int:
+    """How weighted calls add ``weights`` on ``graph`` (jc_weighted_sum_mode):
+    SUM_FP64 when plain fp64 sums are exact, else SUM_DOUBLE_DOUBLE. Synchronous."""
+    _require(weights, _torch().float32, graph.n, "weights", graph.offsets.device)
+    mode = ctypes.c_int(-1)
+    g = graph.c_struct()
+    _check(load_jaccard_lib().jc_weighted_sum_mode(ctypes.byref(g), _ptr(weights), ctypes.byref(mode),
+                                                   _stream(stream)), "jc_weighted_sum_mode")
+    return int(mode.value)
 
 
 def is_symmetric(graph: DeviceCSR, stream=None) -> bool:

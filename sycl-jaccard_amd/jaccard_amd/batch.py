@@ -80,7 +80,9 @@ def batch_specs(count: int = 4096, seed: int = 1, nmin: int = 1 << 8, nmax: int 
     ns = np.floor(np.exp(rng.uniform(math.log(nmin), math.log(nmax), count))).astype(np.int64)
     ns = np.clip(ns, nmin, nmax).astype(np.int32)
     efs = rng.integers(efmin, efmax + 1, count).astype(np.float64)
-    seeds = (np.uint64(seed) * np.uint64(0x9E3779B97F4A7C15) + np.arange(count, dtype=np.uint64)).astype(np.uint64)
+    # seed * golden-ratio constant mod 2^64 (Python integers: no overflow warning)
+    base = np.uint64((int(seed) * 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF)
+    seeds = np.arange(count, dtype=np.uint64) + base
     weighted = (np.arange(count) % 2) == 0
     return ns, efs, seeds, weighted
 
@@ -184,7 +186,12 @@ class BatchRunner:
 
 
 class PlainRunner:
-    """One drop-in jc_jaccard call per graph (local ids), back to back."""
+    """One drop-in jc_jaccard call per graph (local ids), back to back: for a
+    weighted graph jc_vertex_weights then jc_jaccard, as the reference driver
+    does (main.cpp:211-275). The calls go straight to the C ABI with their
+    arguments prepared beforehand, so the loop costs the library's own
+    per-call work (launches, stream-ordered temporaries) plus one ctypes call
+    each, not Python argument checks."""
 
     def __init__(self, batches: List[GraphBatch], device="cuda"):
         torch = api._torch()
@@ -207,9 +214,21 @@ class PlainRunner:
             del uo, ui, uw
         # interleave as generated (global graph id order)
         self.graphs.sort(key=lambda t: int(batches[t[0]].ids[t[1]]))
+        # C-ABI arguments of every call, built once
+        self._calls = []
+        for _, _, dg, out, W in self.graphs:
+            g = dg.c_struct()
+            self._calls.append((g, ctypes.byref(g), out.data_ptr() if dg.m else None,
+                                None if W is None else W.data_ptr()))
 
     def step(self, stream=None):
-        for _, _, dg, out, W in self.graphs:
+        lib = load_jaccard_lib()
+        st = api._stream(stream)
+        for g, gref, out, W in self._calls:
             if W is not None:
-                api.vertex_weights(dg, out=W, stream=stream)
-            api.jaccard(dg, out, weights=W, edge_centric=True, stream=stream)
+                code = lib.jc_vertex_weights(gref, W, st)
+                if code != 0:
+                    raise api.JaccardError(code, "jc_vertex_weights")
+            code = lib.jc_jaccard(gref, W, out, api.EDGE_CENTRIC, st)
+            if code != 0:
+                raise api.JaccardError(code, "jc_jaccard")

@@ -79,3 +79,49 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(_native, "JACCARD_LIB", str(tmp_path / "absent.so"))
     with pytest.raises(_native.NativeLibraryMissing):
         _native.load_jaccard_lib()
+
+
+def _host_graph(n=4, m=6):
+    """A DeviceCSR shell over host tensors: the Python layer must refuse every
+    bad argument before the library is called, so no device is needed."""
+    import torch
+    g = api.DeviceCSR.__new__(api.DeviceCSR)
+    g.n, g.offsets, g.indices, g.edge_w = n, torch.zeros(n + 1, dtype=torch.int32), torch.zeros(m, dtype=torch.int32), None
+    return g
+
+
+@pytest.mark.parametrize("case", ["wrong_dtype", "too_short", "host_tensor", "not_tensor"])
+def test_python_entry_points_check_tensors(case):
+    import torch
+    g = _host_graph()
+    f32 = {"wrong_dtype": torch.zeros(6, dtype=torch.float64), "too_short": torch.zeros(3, dtype=torch.float32),
+           "host_tensor": torch.zeros(6, dtype=torch.float32), "not_tensor": [0.0] * 6}[case]
+    i32 = {"wrong_dtype": torch.zeros(3, dtype=torch.int64), "too_short": torch.zeros(3, dtype=torch.int32),
+           "host_tensor": torch.zeros(3, dtype=torch.int32), "not_tensor": [0] * 3}[case]
+    dest = {"wrong_dtype": torch.zeros(6, dtype=torch.int64), "too_short": torch.zeros(3, dtype=torch.int32),
+            "host_tensor": torch.zeros(6, dtype=torch.int32), "not_tensor": [0] * 6}[case]
+    msg = {"wrong_dtype": "must be torch", "too_short": "at least", "host_tensor": "device",
+           "not_tensor": "torch tensor"}[case]
+    calls = [
+        lambda: api.jaccard(g, result=f32),
+        lambda: api.vertex_weights(g, out=f32),
+        lambda: api.rowsum(g, f32),
+        lambda: api.intersect(g, f32, f32, f32),
+        lambda: api.ec_scan(g, dest, f32),
+        lambda: api.ec_unweighted(g, None, f32),
+    ]
+    if case != "too_short":  # these take their length from the first tensor itself
+        calls += [lambda: api.nonzero_count(f32), lambda: api.fill(f32, 0.0), lambda: api.jw(f32, f32, f32),
+                  lambda: api.jaccard_list(g, i32, i32), lambda: api.csr_from_edges(4, i32, i32)]
+    for call in calls:
+        with pytest.raises(ValueError, match=msg):
+            call()
+
+
+def test_python_weights_checked():
+    import torch
+    g = _host_graph()
+    with pytest.raises(ValueError, match="weights needs at least 4"):
+        api.jaccard(g, result=torch.zeros(6, dtype=torch.float32), weights=torch.zeros(3))
+    with pytest.raises(ValueError, match="same length"):
+        api.jaccard_list(g, torch.zeros(3, dtype=torch.int32), torch.zeros(2, dtype=torch.int32))

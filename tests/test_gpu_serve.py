@@ -103,3 +103,15 @@ def test_budget_gate(setup, cuda):
     assert st["status"][0] == api.ERR_OUT_OF_MEMORY
     assert st["status"][1] == 0
     srv.close()
+
+
+def test_asymmetric_graph_rejected(cuda):
+    """Batched passes write each pair once for both stored directions, so a
+    member without its reverse edges is refused at creation (both modes)."""
+    import cases
+    g = cases.random_directed(n=500, m=3000, seed=9)
+    print(f"[gpu] serve: asymmetric graph n={g.n} m={g.m}", flush=True)
+    for mode in (serve.CONTINUOUS, serve.FIFO):
+        with pytest.raises(api.JaccardError) as ex:
+            serve.Server([g], cuda, weighted=False, mode=mode)
+        assert ex.value.code == api.ERR_INVALID_CSR

@@ -110,7 +110,7 @@ typedef struct jc_tuning {
                                   them); for testing both load paths */
   int32_t bitmap_blocks_per_cu; /* workgroups per CU asked for the bitmap-table task
                                   kernel (1 or 2, default 1). With 2 its LDS table
-                                  region shrinks to 48 KiB (rows whose bitmap no longer
+                                  region shrinks to 32 KiB (rows whose bitmap no longer
                                   fits use the hash-table kernel); the grid is the CU
                                   count times what the kernel's registers and LDS let
                                   reside, at most this value */

@@ -348,7 +348,7 @@ struct QTable {
       const bool m1 = e1 | e3 | e5, m2 = e2 | e3 | e6, m4 = e4 | e5 | e6;
       const uint32_t slot = (m1 ? 1u : 0u) + (m2 ? 2u : 0u) + (m4 ? 4u : 0u);
       const uint32_t base = __builtin_amdgcn_ubfe(w.w, 16, 15);
-      h.v = val[lane_select(h.hit, 1u + base + slot, 0u)];
+      h.v = val[h.hit ? 1u + base + slot : 0u];  // one v_cndmask on the hit mask
     }
     return h;
   }
@@ -693,16 +693,18 @@ __device__ __forceinline__ void probe_q(const Chunk<W>& cx, const int32_t (&y)[U
 
 // Probes the U entries of one pass per lane (bitmap or hash table). COUNT:
 // also counts the entries below u (u's position in the row, the mirror
-// position) and collects the lanes that hold u in the wave mask `seen` (one
-// compare into a scalar mask per entry).
+// position; whether u is there is checked when the pair is emitted): one
+// compare per entry, counted per lane, or for whole-wave groups by the
+// scalar unit from the compare's lane mask.
 template <bool W, int G, int U, bool FAST, bool COUNT, bool STOP, int TK, bool FA>
-__device__ __forceinline__ void probe_pass(const Chunk<W>& cx, int32_t (&y)[U], int& cnt, DD& acc, int& below,
-                                           uint64_t& seen) {
+__device__ __forceinline__ void probe_pass(const Chunk<W>& cx, int32_t (&y)[U], int& cnt, DD& acc, int& below) {
   if constexpr (COUNT) {
 #pragma unroll
     for (int t = 0; t < U; ++t) {
-      below += y[t] < cx.u ? 1 : 0;
-      seen |= ballot(y[t] == cx.u);
+      if constexpr (G == kWave)
+        below += popc64(ballot(y[t] < cx.u));
+      else
+        below += y[t] < cx.u ? 1 : 0;
     }
   }
   if constexpr (TK == kTkBitmap)
@@ -770,7 +772,7 @@ __device__ __forceinline__ void load_pass(const EngineArgs& a, int32_t (&y)[U], 
 // the stop position, or e.
 template <bool W, int G, int U, bool FAST, bool COUNT, bool STOP, int TK, bool ONEPASS, bool BUF, bool FA>
 __device__ __forceinline__ int stream_t(const EngineArgs& a, const Chunk<W>& cx, int s, int e, int& cnt, DD& acc,
-                                        int& below, uint64_t& seen) {
+                                        int& below) {
   const int gl = lane_id() & (G - 1);
   int p0 = s;
   if (p0 >= e) return e;
@@ -787,7 +789,7 @@ __device__ __forceinline__ int stream_t(const EngineArgs& a, const Chunk<W>& cx,
     while (p0 + G * U < e) {
       int32_t yn[U];
       load_pass<BUF, G, U>(a, yn, p0 + G * U, e);
-      probe_pass<W, G, U, FAST, COUNT, STOP, TK, FA>(cx, y, cnt, acc, below, seen);
+      probe_pass<W, G, U, FAST, COUNT, STOP, TK, FA>(cx, y, cnt, acc, below);
       if constexpr (STOP) {
         const int st = pass_stop<G, U>(y, cx.hi, p0, e);
         if (st >= 0) return st;
@@ -802,7 +804,7 @@ __device__ __forceinline__ int stream_t(const EngineArgs& a, const Chunk<W>& cx,
     for (;;) {
       if (p0 + G * U >= e) break;
       load_pass<BUF, G, U>(a, yb, p0 + G * U, e);
-      probe_pass<W, G, U, FAST, COUNT, STOP, TK, FA>(cx, y, cnt, acc, below, seen);
+      probe_pass<W, G, U, FAST, COUNT, STOP, TK, FA>(cx, y, cnt, acc, below);
       if constexpr (STOP) {
         const int st = pass_stop<G, U>(y, cx.hi, p0, e);
         if (st >= 0) return st;
@@ -814,7 +816,7 @@ __device__ __forceinline__ int stream_t(const EngineArgs& a, const Chunk<W>& cx,
         break;
       }
       load_pass<BUF, G, U>(a, y, p0 + G * U, e);
-      probe_pass<W, G, U, FAST, COUNT, STOP, TK, FA>(cx, yb, cnt, acc, below, seen);
+      probe_pass<W, G, U, FAST, COUNT, STOP, TK, FA>(cx, yb, cnt, acc, below);
       if constexpr (STOP) {
         const int st = pass_stop<G, U>(yb, cx.hi, p0, e);
         if (st >= 0) return st;
@@ -825,7 +827,7 @@ __device__ __forceinline__ int stream_t(const EngineArgs& a, const Chunk<W>& cx,
 #pragma unroll
   for (int t = 0; t < U; ++t)
     if (p0 + t * G + gl >= e) y[t] = INT32_MAX;
-  probe_pass<W, G, U, FAST, COUNT, STOP, TK, FA>(cx, y, cnt, acc, below, seen);
+  probe_pass<W, G, U, FAST, COUNT, STOP, TK, FA>(cx, y, cnt, acc, below);
   if constexpr (STOP) {
     const int st = pass_stop<G, U>(y, cx.hi, p0, e);
     if (st >= 0) return st;
@@ -835,10 +837,47 @@ __device__ __forceinline__ int stream_t(const EngineArgs& a, const Chunk<W>& cx,
 
 template <bool W, int G, int U, bool FAST, bool COUNT, bool STOP, int TK, bool ONEPASS, bool FA>
 __device__ __forceinline__ int stream(const EngineArgs& a, const Chunk<W>& cx, int s, int e, int& cnt, DD& acc,
-                                      int& below, uint64_t& seen) {
+                                      int& below) {
   if (!a.wide_index)
-    return stream_t<W, G, U, FAST, COUNT, STOP, TK, ONEPASS, true, FA>(a, cx, s, e, cnt, acc, below, seen);
-  return stream_t<W, G, U, FAST, COUNT, STOP, TK, ONEPASS, false, FA>(a, cx, s, e, cnt, acc, below, seen);
+    return stream_t<W, G, U, FAST, COUNT, STOP, TK, ONEPASS, true, FA>(a, cx, s, e, cnt, acc, below);
+  return stream_t<W, G, U, FAST, COUNT, STOP, TK, ONEPASS, false, FA>(a, cx, s, e, cnt, acc, below);
+}
+
+// A unit's result for pair slot `slot`, kept in LDS until the task's emits
+// (emit_deferred): the count (unweighted) or the exact sum (weighted, its
+// leading double), and the mirror position (-1: u not in row x). The slots
+// are the multi-chunk partial and cursor arrays, which a unit reads before it
+// writes its own slot.
+template <bool W>
+__device__ __forceinline__ void defer_result(const Chunk<W>& cx, int slot, uint32_t cnt, const DD& acc, int64_t q) {
+  if constexpr (W)
+    cx.pacc[slot].hi = acc.hi;
+  else
+    cx.pcnt[slot] = cnt;
+  cx.cursor[slot] = (int)q;
+}
+
+// The emits of a task's (or row's) pairs, one thread per pair once every unit
+// has stored its result: the division and the two stores run with all lanes
+// busy instead of one lane per unit. Pair i is row x = ix[i] (degree
+// idx_[i]) at entry ij[i] of the owner row u, whose entry 0 is at kbase.
+template <int MODE, bool W>
+__device__ __forceinline__ void emit_deferred(const EngineArgs& a, int32_t u, int du, double su, int64_t kbase,
+                                              int i, int32_t x, int dx, int j, const int* rq,
+                                              const uint32_t* rcnt, const DD* racc) {
+  const double sx = row_sum_of<MODE>(a, x);
+  DD acc{0.0, 0.0};
+  uint32_t cnt = 0;
+  if constexpr (W)
+    acc.hi = racc[i].hi;
+  else
+    cnt = rcnt[i];
+  // the mirror position: entry `below` of row x, if that entry is u (a CSR
+  // that is not symmetric may lack (x, u); that position is left to the
+  // fix-up pass)
+  int64_t q = rq[i];
+  if (q >= 0 && a.ind[q] != u) q = -1;
+  emit<MODE, W>(a, u, du, x, dx, kbase + j, q, cnt, acc, su, sx);
 }
 
 // One unit of work: each group of G lanes handles one owned short row x
@@ -859,25 +898,19 @@ __device__ __forceinline__ void unit(const EngineArgs& a, const Chunk<W>& cx, bo
   int s = ox;
   if constexpr (MULTI) s = (valid && cx.c > 0) ? cx.cursor[item] : ox;
   int e = valid ? end : s;
-  // the epilogue's row sum of x, loaded while the row streams
-  double sx = 0.0;
-  if constexpr (MODE != kModeEcU) {
-    if (valid && gl == 0) sx = row_sum_of<MODE>(a, x);
-  }
   // single-chunk units count the entries below u while streaming (the mirror
   // position); multi-chunk units locate u with one group search at the end
   constexpr bool kCountMirror = !MULTI && (JC_MIRRORCOUNT || G * U <= 64);
   constexpr bool kOnePass = G * U <= 64;
   constexpr bool kFetchAll = JC_FETCHALL && !SMALL;  // probe_u
   int cnt = 0, below = 0;
-  uint64_t seen = 0;
   DD acc{0.0, 0.0};
   int stop_at = e;
   if (MULTI && !last)
-    stop_at = stream<W, G, U, FAST, kCountMirror, true, TK, false, kFetchAll>(a, cx, s, e, cnt, acc, below, seen);
+    stop_at = stream<W, G, U, FAST, kCountMirror, true, TK, false, kFetchAll>(a, cx, s, e, cnt, acc, below);
   else
     stream<W, G, U, FAST, kCountMirror, false, TK, SMALL || (!MULTI && kOnePass), kFetchAll>(a, cx, s, e, cnt, acc,
-                                                                                            below, seen);
+                                                                                            below);
   (void)gm;
   (void)gl;
   if constexpr (kCountMirror && kOnePass) {
@@ -887,7 +920,7 @@ __device__ __forceinline__ void unit(const EngineArgs& a, const Chunk<W>& cx, bo
     below = packed >> 8;
   } else if constexpr (kCountMirror) {
     cnt = grp_sum_i<G>(cnt);
-    below = grp_sum_i<G>(below);
+    if constexpr (G != kWave) below = grp_sum_i<G>(below);  // whole-wave counts are totals already
   } else {
     cnt = grp_sum_i<G>(cnt);
   }
@@ -900,14 +933,14 @@ __device__ __forceinline__ void unit(const EngineArgs& a, const Chunk<W>& cx, bo
   if constexpr (!MULTI) {
     int64_t q = -1;
     if constexpr (kCountMirror) {
-      // u sits at entry `below` of row x if a lane of the group saw it
-      if ((seen & gm) != 0) q = (int64_t)ox + below;
+      // u sits at entry `below` of row x if it is in the row (the emit checks)
+      if (below < dx) q = (int64_t)ox + below;
     } else {
       int pos = end;
       if (valid) pos = grp_lower_bound<G>(a.ind, ox, end, cx.u);
       if (valid && pos < end && a.ind[pos] == cx.u) q = pos;
     }
-    if (valid && gl == 0) emit<MODE, W>(a, cx.u, cx.du, x, dx, cx.kbase + j, q, (uint32_t)cnt, acc, cx.su, sx);
+    if (valid && gl == 0) defer_result<W>(cx, SMALL ? j : item, (uint32_t)cnt, acc, q);
   } else {
     if (!last) {
       if (valid && gl == 0) {
@@ -936,7 +969,7 @@ __device__ __forceinline__ void unit(const EngineArgs& a, const Chunk<W>& cx, bo
           tot += cx.pcnt[item];
         }
         const bool f = q < end && a.ind[q] == cx.u;
-        emit<MODE, W>(a, cx.u, cx.du, x, dx, cx.kbase + j, f ? (int64_t)q : -1, tot, acc, cx.su, sx);
+        defer_result<W>(cx, item, tot, acc, f ? (int64_t)q : -1);
       }
     }
   }
@@ -1057,6 +1090,10 @@ __global__ __launch_bounds__(BLOCK) void k_owner_small(EngineArgs a) {
   constexpr int SLOTS = kBucket << 6;  // 64 buckets hold a row of up to 128 keys at <= 2 per bucket
   __shared__ __align__(16) int32_t key_all[NW][SLOTS];
   __shared__ __align__(16) float val_all[W ? NW : 1][W ? SLOTS : 1];
+  // per-wave results of a row's pairs until the row's emits, by entry index
+  __shared__ int s_rq[NW][kSmallMax];
+  __shared__ __align__(16) uint32_t s_rcnt[W ? 1 : NW][W ? 1 : kSmallMax];
+  __shared__ __align__(16) DD s_racc[W ? NW : 1][W ? kSmallMax : 1];
   const int w = threadIdx.x / kWave;
   const int lane = lane_id();
   const int nsmall = *a.num_small;
@@ -1096,12 +1133,18 @@ __global__ __launch_bounds__(BLOCK) void k_owner_small(EngineArgs a) {
     cx.c = 0;
     cx.nch = 1;
     cx.kbase = ou;
-    cx.cursor = nullptr;
-    cx.pcnt = nullptr;
-    cx.pacc = nullptr;
+    cx.cursor = s_rq[w];
+    cx.pcnt = W ? nullptr : s_rcnt[W ? 0 : w];
+    cx.pacc = W ? s_racc[W ? w : 0] : nullptr;
     run_batch_regs<MODE, W>(a, cx, c0, x0, ox0, dx0, lane, fast);
     run_batch_regs<MODE, W>(a, cx, c1, x1, ox1, dx1, lane + kWave, fast);
-    lds_fence();  // all probes of this row done before the next row clears the table
+    lds_fence();  // the units' results are visible to every lane
+    // each lane emits the pairs of its own entries (j = lane, lane + 64)
+    if (c0 && owns(u, du, x0, dx0))
+      emit_deferred<MODE, W>(a, u, du, cx.su, ou, lane, x0, dx0, lane, cx.cursor, cx.pcnt, cx.pacc);
+    if (c1 && owns(u, du, x1, dx1))
+      emit_deferred<MODE, W>(a, u, du, cx.su, ou, lane + kWave, x1, dx1, lane + kWave, cx.cursor, cx.pcnt, cx.pacc);
+    lds_fence();  // all probes and emits of this row done before the next row clears the table
   }
 }
 
@@ -1151,8 +1194,9 @@ __device__ __forceinline__ int block_excl_scan(int v, int* s_wsum) {
 }
 
 // Table region of the bitmap-task kernel when two workgroups share a CU:
-// 2 x (48 KiB + item list + staging + static LDS) fits the 160 KiB of a CU.
-constexpr size_t kBitmapLds2 = 48 * 1024;
+// 2 x (32 KiB + item list + staging + results + static LDS) fits the 160 KiB
+// of a CU.
+constexpr size_t kBitmapLds2 = 32 * 1024;
 
 template <bool W>
 struct BlockCfg {
@@ -1317,8 +1361,12 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
   int32_t* iox = ix + kTaskSpan;
   int32_t* idx_ = iox + kTaskSpan;
   uint16_t* ij = reinterpret_cast<uint16_t*>(idx_ + kTaskSpan);
-  int* cursor = KIND == 1 ? nullptr : reinterpret_cast<int*>(items + Cfg::item_bytes);
-  unsigned char* partp = KIND == 1 ? nullptr : items + Cfg::item_bytes + Cfg::cursor_bytes;
+  // multi-chunk cursors and partials (hash-table tasks), which also hold the
+  // units' results until the task's emits (all tasks); bitmap tasks keep them
+  // after their staging area
+  unsigned char* resp = items + Cfg::item_bytes + (KIND == 1 ? Cfg::stage_bytes : 0);
+  int* cursor = reinterpret_cast<int*>(resp);
+  unsigned char* partp = resp + Cfg::cursor_bytes;
   uint32_t* pcnt = reinterpret_cast<uint32_t*>(partp);
   DD* pacc = reinterpret_cast<DD*>(partp);
   // Bitmap tasks look one task ahead: while the other waves run a task's
@@ -1557,10 +1605,13 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
         run_unit_lds<MODE, W, false, kTkBitmap>(a, cx, cls_k, first, s_cbase[k + 1], ix, iox, idx_, ij, fast);
       }
       __syncthreads();
+      for (int i = tid; i < s_cbase[kClasses]; i += BLOCK)
+        emit_deferred<MODE, W>(a, u, du, cx.su, ka, i, ix[i], idx_[i], ij[i], cursor, pcnt, pacc);
       continue;
     }
     if constexpr (KIND != 1) {
       constexpr int TK = kQ ? kTkQuot : kTkHash;
+      bool built = true;  // false: the task was handed to the 32-bit redo pass
       for (int c = 0; c < nch; ++c) {
         const int cs = ou + c * CH;
         const int ce = min(cs + CH, ou + du);
@@ -1582,6 +1633,7 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
           if (s_fail) {
             // nothing of this task has been written yet: the 32-bit table redoes it
             if (tid == 0) a.redo[atomicAdd(a.num_redo, 1)] = t;
+            built = false;
             break;
           }
         } else {
@@ -1624,6 +1676,11 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
             run_unit_lds<MODE, W, true, TK>(a, cx, cls_k, first, cend, ix, iox, idx_, ij, fast);
         }
         __syncthreads();
+      }
+      if (built) {
+        const double su = row_sum_of<MODE>(a, u);
+        for (int i = tid; i < s_cbase[kClasses]; i += BLOCK)
+          emit_deferred<MODE, W>(a, u, du, su, ka, i, ix[i], idx_[i], ij[i], cursor, pcnt, pacc);
       }
     }
   }
@@ -1889,13 +1946,12 @@ hipError_t run_engine_t(const EngineArgs& base, void* ws, bool fixup, hipStream_
   if (rows > 0) {
     const int g = std::min<int64_t>((rows + PB - 1) / PB, (int64_t)cus * 8);
     using Cfg = BlockCfg<W>;
-    // bitmap tasks (single-chunk: no cursors or partials) take all of the
-    // LDS but the item list and the staging area with one workgroup per CU,
-    // or 48 KiB with two
+    // bitmap tasks take all of the LDS but the item list, the staging area
+    // and the result slots with one workgroup per CU, or 32 KiB with two
     const int bm_per_cu = a.bitmap_blocks_per_cu == 2 ? 2 : 1;
-    const size_t bm_budget =
-        bm_per_cu == 2 ? kBitmapLds2 : kBlockLds - Cfg::item_bytes - Cfg::stage_bytes;
-    const size_t lds1 = bm_budget + Cfg::item_bytes + Cfg::stage_bytes;
+    const size_t bm_rest = Cfg::item_bytes + Cfg::stage_bytes + Cfg::cursor_bytes + Cfg::part_bytes;
+    const size_t bm_budget = bm_per_cu == 2 ? kBitmapLds2 : kBlockLds - bm_rest;
+    const size_t lds1 = bm_budget + bm_rest;
     a.bm_budget = (uint32_t)bm_budget;
     {
       ProfScope ps("sched", stream);

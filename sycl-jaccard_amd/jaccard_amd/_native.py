@@ -136,6 +136,12 @@ def load_jaccard_lib() -> ctypes.CDLL:
     global _jaccard
     with _lock:
         if _jaccard is None:
+            # torch carries its own HIP runtime (libamdhip64.so, under a SONAME
+            # other than ROCm's libamdhip64.so.7). Loaded first, it is the one
+            # the library binds to; loaded after the library, the process would
+            # hold two HIP runtimes, and torch's memory and streams would
+            # belong to the other one.
+            import torch  # noqa: F401
             lib = _load(JACCARD_LIB)
             _declare_jaccard(lib)
             _jaccard = lib

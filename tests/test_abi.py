@@ -32,6 +32,23 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_one_hip_runtime_in_process():
+    """Loading the library before anything imported torch still leaves ONE
+    HIP runtime in the process (torch's): a second runtime would own none of
+    torch's tensors or streams."""
+    import subprocess
+    import sys
+    code = (
+        "import sys; sys.path.insert(0, %r)\n"
+        "from jaccard_amd import _native\n"
+        "_native.load_jaccard_lib()\n"
+        "import torch; torch.cuda.is_available()\n"
+        "maps = {l.split()[-1] for l in open('/proc/self/maps') if 'libamdhip64' in l}\n"
+        "print(len(maps)); print(*sorted(maps), sep='\\n')\n" % os.path.join(ROOT, "sycl-jaccard_amd"))
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, check=True).stdout
+    assert out.splitlines()[0] == "1", out
+
+
 def test_header_compiles_as_c():
     import subprocess
     import tempfile

@@ -51,7 +51,9 @@ enum {
   JC_ERR_INVALID_CSR = -2,    /* jc_validate_csr found a violated precondition */
   JC_ERR_OUT_OF_MEMORY = -3,  /* device allocation failed */
   JC_ERR_HIP = -4,            /* a HIP runtime call failed, see jc_last_hip_error() */
-  JC_ERR_WORKSPACE = -5       /* supplied workspace smaller than required */
+  JC_ERR_WORKSPACE = -5,      /* supplied workspace smaller than required */
+  JC_ERR_SCRATCH_LIMIT = -6   /* jc_jaccard_noalloc: the graph needs more than
+                                 JC_NOALLOC_SCRATCH_BYTES of workspace */
 };
 
 /* Graph in CSR form: GraphCSRView<int32_t,int32_t,float>
@@ -124,6 +126,21 @@ typedef struct jc_tuning {
                                   chunk with a full bucket) */
   int32_t reserved[2];
 } jc_tuning;
+
+/* jc_jaccard_ex that allocates nothing, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, hosts
+ * that forbid allocation). No reference counterpart. Its workspace is a
+ * fixed array of JC_NOALLOC_SCRATCH_BYTES that the library holds on every
+ * device it runs on until the process ends; a graph whose
+ * jc_jaccard_workspace_size() exceeds it gives JC_ERR_SCRATCH_LIMIT (32 MiB:
+ * about 1.3M vertices edge-centric weighted, 2.6M unweighted; vertex-centric
+ * also needs 8 or 16 bytes per stored edge). Inputs and the result are the
+ * caller's; the result is the bits of jc_jaccard_ex. All calls on one device
+ * share that device's scratch: calls that overlap on one device (other
+ * streams or threads) race and must be serialized by the caller. */
+#define JC_NOALLOC_SCRATCH_BYTES (32u << 20)
+int jc_jaccard_noalloc(const jc_csr *g, const float *vertex_weights, float *result, int edge_centric,
+                       unsigned flags, jc_stream_t stream);
 
 /* jc_jaccard_ex with launch tuning (NULL = defaults). */
 int jc_jaccard_tuned(const jc_csr *g, const float *vertex_weights, float *result, int edge_centric,

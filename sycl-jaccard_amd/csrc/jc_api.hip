@@ -8,6 +8,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "../../include/jaccard_amd.h"
 #include "jc_device.hpp"
@@ -137,6 +138,35 @@ int run_jaccard(const jc_csr* g, const float* vw, float* result, bool ec, unsign
   return JC_OK;
 }
 
+// Scratch of jc_jaccard_noalloc: one array per device, held until the process
+// ends. In host code `scratch` names a host-side stand-in, never the device
+// memory: the device address comes from hipGetSymbolAddress, once per device
+// (with that device current), and only that address is handed to kernels.
+static __device__ __align__(256) unsigned char scratch[JC_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the current device's scratch array (resolved on the
+// first call on that device), or an error.
+hipError_t scratch_address(void** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return hipSuccess;
+}
+
 }  // namespace
 
 namespace jc {
@@ -178,6 +208,20 @@ int jc_jaccard_ex(const jc_csr* g, const float* vertex_weights, float* result, i
 int jc_jaccard(const jc_csr* g, const float* vertex_weights, float* result, int edge_centric,
                jc_stream_t stream) {
   return jc_jaccard_ex(g, vertex_weights, result, edge_centric, 0u, nullptr, 0, stream);
+}
+
+int jc_jaccard_noalloc(const jc_csr* g, const float* vertex_weights, float* result, int edge_centric,
+                       unsigned flags, jc_stream_t stream) {
+  if (!csr_args_ok(g) || (g->m > 0 && !result) || (flags & ~JC_FLAG_ASSUME_SYMMETRIC)) return JC_ERR_INVALID_ARG;
+  if (g->n == 0 || g->m == 0) return JC_OK;
+  const bool ec = edge_centric != 0;
+  if (layout(g->n, g->m, vertex_weights != nullptr, ec).total > (size_t)JC_NOALLOC_SCRATCH_BYTES)
+    return JC_ERR_SCRATCH_LIMIT;
+  void* ws = nullptr;
+  JC_TRY(scratch_address(&ws));
+  // Every part of the workspace that must start at zero (schedule counters,
+  // range facts, row-sum queue) is cleared by the kernels of the call itself.
+  return run_jaccard(g, vertex_weights, result, ec, flags, static_cast<unsigned char*>(ws), nullptr, hs(stream));
 }
 
 int jc_jaccard_list(const jc_csr* g, const float* vertex_weights, int32_t num_pairs,
@@ -377,6 +421,7 @@ const char* jc_strerror(int code) {
     case JC_ERR_OUT_OF_MEMORY: return "device out of memory";
     case JC_ERR_HIP: return "HIP runtime error";
     case JC_ERR_WORKSPACE: return "workspace too small";
+    case JC_ERR_SCRATCH_LIMIT: return "graph too large for the allocation-free entry point's fixed scratch";
     default: return "unknown error";
   }
 }

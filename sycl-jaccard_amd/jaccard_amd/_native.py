@@ -97,6 +97,7 @@ def _declare_jaccard(lib: ctypes.CDLL) -> None:
         "jc_jaccard_ex": [vp, vp, vp, I, u32, vp, sz, vp],
         "jc_jaccard_workspace_size": [i32, i32, I, I],
         "jc_jaccard_tuned": [vp, vp, vp, I, u32, vp, sz, vp, vp],
+        "jc_jaccard_noalloc": [vp, vp, vp, I, u32, vp],
         "jc_jaccard_list": [vp, vp, i32, vp, vp, vp, vp],
         "jc_vertex_weights": [vp, vp, vp],
         "jc_vertex_weights_ex": [vp, vp, vp, sz, vp],

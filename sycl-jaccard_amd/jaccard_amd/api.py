@@ -174,6 +174,32 @@ def jaccard(graph: DeviceCSR, result=None, weights=None, edge_centric: bool = Tr
     return result
 
 
+ERR_SCRATCH_LIMIT = -6
+
+
+def jaccard_noalloc(graph: DeviceCSR, result=None, weights=None, edge_centric: bool = True, stream=None,
+                    flags: int = 0):
+    """``jaccard`` through ``jc_jaccard_noalloc``: the library allocates no
+    device memory (its workspace is a fixed per-device array of
+    ``NOALLOC_SCRATCH_BYTES``); graphs that need more raise JaccardError
+    ERR_SCRATCH_LIMIT. Overlapping calls on one device must be serialized."""
+    torch = _torch()
+    lib = load_jaccard_lib()
+    dev = graph.offsets.device
+    if result is None:
+        result = _empty_f32(graph.m, dev)
+    _require(weights, torch.float32, graph.n, "weights", dev, optional=True)
+    _require(result, torch.float32, graph.m, "result", dev)
+    g = graph.c_struct()
+    algo = EDGE_CENTRIC if edge_centric else VERTEX_CENTRIC
+    _check(lib.jc_jaccard_noalloc(ctypes.byref(g), _ptr(weights), _ptr(result) if graph.m else None, algo, flags,
+                                  _stream(stream)), "jc_jaccard_noalloc")
+    return result
+
+
+NOALLOC_SCRATCH_BYTES = 32 << 20
+
+
 def workspace_size(n: int, m: int, weighted: bool, edge_centric: bool) -> int:
     return int(load_jaccard_lib().jc_jaccard_workspace_size(n, m, int(weighted), int(edge_centric)))
 

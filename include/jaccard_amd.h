@@ -133,8 +133,9 @@ typedef struct jc_tuning {
  * fixed array of JC_NOALLOC_SCRATCH_BYTES that the library holds on every
  * device it runs on until the process ends; a graph whose
  * jc_jaccard_workspace_size() exceeds it gives JC_ERR_SCRATCH_LIMIT (32 MiB:
- * about 1.3M vertices edge-centric weighted, 2.6M unweighted; vertex-centric
- * also needs 8 or 16 bytes per stored edge). Inputs and the result are the
+ * about 1.18M vertices edge-centric weighted, 2.05M unweighted, at 16 to 30
+ * stored edges per vertex; vertex-centric also needs 8 or 16 bytes per
+ * stored edge). Inputs and the result are the
  * caller's; the result is the bits of jc_jaccard_ex. All calls on one device
  * share that device's scratch: calls that overlap on one device (other
  * streams or threads) race and must be serialized by the caller. */
@@ -146,6 +147,55 @@ int jc_jaccard_noalloc(const jc_csr *g, const float *vertex_weights, float *resu
 int jc_jaccard_tuned(const jc_csr *g, const float *vertex_weights, float *result, int edge_centric,
                      unsigned flags, void *workspace, size_t workspace_bytes, const jc_tuning *tuning,
                      jc_stream_t stream);
+
+/* Automatic configuration from graph properties, the counterpart of the
+ * reference driver's "TODO: Automatic selection based on graph properties"
+ * (main.cpp:203), which there picks nothing (one of the FORCE variables must
+ * be set, main.cpp:48-61).
+ *
+ * jc_degree_stats (synchronous) gathers on the device: rows[b] and
+ * entries[b], the rows of degree in [2^b, 2^(b+1)) and their stored entries;
+ * isolated rows; long_rows, the rows above the one-wave-per-row bin (degree
+ * > 128); long_bitmap_rows[w], those among them that the workgroup kernel
+ * gives a bitmap table (w = 0 unweighted, 1 weighted layout, one workgroup
+ * per CU): a row with first and last column lo, hi and degree d gets one when
+ * W = (hi - lo) / 32 + 2 words of 4 bytes (weighted: W words of 8 bytes plus
+ * 8 * (d + 1) bytes of weights) fit bitmap_table_bytes[w]; the largest
+ * degree. */
+typedef struct jc_graph_stats {
+  int64_t rows[32];
+  int64_t entries[32];
+  int64_t isolated;
+  int64_t long_rows;
+  int64_t long_bitmap_rows[2];
+  int32_t max_degree;
+  int32_t reserved;
+  int64_t bitmap_table_bytes[2];
+} jc_graph_stats;
+int jc_degree_stats(const jc_csr *g, jc_graph_stats *stats, jc_stream_t stream);
+
+/* The algorithm and launch tuning for a graph with statistics *stats and n
+ * vertices (host only, no device work; *edge_centric and *tuning are
+ * outputs). The rule:
+ *   - edge-centric always: both algorithms give the same values from the same
+ *     sums, and the vertex-centric one also writes and reads two m-length
+ *     arrays (weight_i, weight_s) and runs the jw pass;
+ *   - small_max 128: every row up to the largest the one-wave-per-row
+ *     kernel's table holds goes to that kernel (smaller bins were slower in
+ *     every family measured, up to 2.4x);
+ *   - small_blocks_per_cu 32 (best or within 1% in every family);
+ *   - bitmap_blocks_per_cu 1 (two workgroups per CU shrink the bitmap table
+ *     and were up to 6% slower);
+ *   - hash_table 0 (quotient tables) below 2^24 vertices, 1 (32-bit tables)
+ *     from there on.
+ * These come from a sweep of every jc_tuning field over seven graph families
+ * (RMAT 2^16 to 2^22, weighted and not, Chung-Lu 2^20, the two halves of a
+ * 4096-graph batch); no field changes results. */
+int jc_config_from_stats(const jc_graph_stats *stats, int32_t n, int weighted, int *edge_centric,
+                         jc_tuning *tuning);
+
+/* jc_degree_stats then jc_config_from_stats (synchronous). */
+int jc_select_config(const jc_csr *g, int weighted, int *edge_centric, jc_tuning *tuning, jc_stream_t stream);
 
 /* sygraph::jaccard_list<int, int, float>(graph, [weights,] num_pairs, first, second, result, q)
  *   declarations standalone_algorithms.hpp:109-112, :131-133; definitions

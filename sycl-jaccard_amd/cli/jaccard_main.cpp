@@ -18,9 +18,12 @@
 // <c>" (jaccard.cpp:811-819); the vertex-centric path on a weighted graph
 // derives vertex weights from the edge weights first (main.cpp:211-275).
 // With neither FORCE variable the reference runs no kernel at all and writes
-// uninitialised results (main.cpp:204-209); here the choice is automatic:
-// vertex-centric for weighted graphs, edge-centric otherwise (the two give
-// identical bits on unweighted graphs).
+// uninitialised results (main.cpp:204-209, "TODO: Automatic selection based
+// on graph properties" at :203); here jc_select_config picks the algorithm
+// and launch tuning from the graph's degree statistics, and the quantity
+// follows the input: weighted Jaccard (vertex weights from the edge weights)
+// for a weighted graph, unweighted otherwise. Both algorithms give the same
+// bits for the same quantity.
 //
 // Exit codes: 1 bad input extension, 2 bad output extension, 3 binary CSR of
 // other types, MTX errors as read_mtx, 4 device or library failure.
@@ -176,11 +179,7 @@ int main(int argc, char** argv) {
     vc = true;
     ec = false;
   }
-  if (!ec && !vc) {
-    vc = weighted;
-    ec = !weighted;
-    std::cerr << "AUTO " << (ec ? "Edge-Centric" : "Vertex-Centric") << " Implementation" << std::endl;
-  }
+  const bool autosel = !ec && !vc;
   const bool profile = std::getenv("JACCARD_EVENT_PROFILE") != nullptr;
   if (profile) jc_profile_enable(1);
 
@@ -190,20 +189,35 @@ int main(int argc, char** argv) {
   float* d_out = nullptr;
   check_hip(hipMalloc(&d_out, std::max<size_t>((size_t)m, 1) * sizeof(float)), "hipMalloc");
   jc_csr c{n, m, d_off, d_ind, d_ew};
+  jc_tuning tuning{};
+  if (autosel) {
+    int e = JC_EDGE_CENTRIC;
+    check_jc(jc_select_config(&c, weighted ? 1 : 0, &e, &tuning, nullptr), "jc_select_config");
+    ec = e == JC_EDGE_CENTRIC;
+    vc = !ec;
+    std::cerr << "AUTO " << (ec ? "Edge-Centric" : "Vertex-Centric") << " Implementation"
+              << (weighted ? " (weighted)" : "") << " small_max " << tuning.small_max << " small_blocks_per_cu "
+              << tuning.small_blocks_per_cu << " bitmap_blocks_per_cu " << tuning.bitmap_blocks_per_cu
+              << " hash_table " << tuning.hash_table << std::endl;
+  }
+  // vertex weights: the weighted quantity of the vertex-centric path, and of
+  // the automatic choice on a weighted graph (a forced edge-centric run is
+  // unweighted, as in the reference)
+  float* d_w = nullptr;
+  if (weighted && (vc || autosel)) {
+    check_hip(hipMalloc(&d_w, std::max<size_t>((size_t)n, 1) * sizeof(float)), "hipMalloc");
+    check_jc(jc_vertex_weights(&c, d_w, nullptr), "jc_vertex_weights");
+  }
+  const jc_tuning* tp = autosel ? &tuning : nullptr;
   if (ec) {
-    check_jc(jc_jaccard(&c, nullptr, d_out, JC_EDGE_CENTRIC, nullptr), "jc_jaccard");
+    check_jc(jc_jaccard_tuned(&c, d_w, d_out, JC_EDGE_CENTRIC, 0u, nullptr, 0, tp, nullptr), "jc_jaccard");
     int64_t nz = 0;
     check_jc(jc_nonzero_count(m, d_out, 1e-5f, &nz, nullptr), "jc_nonzero_count");
     std::cout << "vertices " << n << "edges " << m << "non zero pairs " << nz << std::endl;
-  } else if (weighted) {
-    float* d_w = nullptr;
-    check_hip(hipMalloc(&d_w, std::max<size_t>((size_t)n, 1) * sizeof(float)), "hipMalloc");
-    check_jc(jc_vertex_weights(&c, d_w, nullptr), "jc_vertex_weights");
-    check_jc(jc_jaccard(&c, d_w, d_out, JC_VERTEX_CENTRIC, nullptr), "jc_jaccard");
-    check_hip(hipFree(d_w), "hipFree");
   } else {
-    check_jc(jc_jaccard(&c, nullptr, d_out, JC_VERTEX_CENTRIC, nullptr), "jc_jaccard");
+    check_jc(jc_jaccard_tuned(&c, d_w, d_out, JC_VERTEX_CENTRIC, 0u, nullptr, 0, tp, nullptr), "jc_jaccard");
   }
+  if (d_w) check_hip(hipFree(d_w), "hipFree");
   std::vector<float> res((size_t)m);
   if (m) check_hip(hipMemcpy(res.data(), d_out, (size_t)m * sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy");
   check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");

@@ -371,6 +371,55 @@ int jc_weighted_sum_mode(const jc_csr* g, const float* vertex_weights, int* mode
   return JC_OK;
 }
 
+int jc_degree_stats(const jc_csr* g, jc_graph_stats* stats, jc_stream_t stream) {
+  if (!csr_args_ok(g) || !stats) return JC_ERR_INVALID_ARG;
+  *stats = jc_graph_stats{};
+  jc::bitmap_table_bytes(stats->bitmap_table_bytes);
+  if (g->n == 0) return JC_OK;
+  hipStream_t s = hs(stream);
+  constexpr int kWords = 69;  // see launch_degree_stats
+  AsyncBuf buf;
+  buf.s = s;
+  JC_TRY(hipMallocAsync(&buf.p, kWords * sizeof(unsigned long long), s));
+  auto* d = static_cast<unsigned long long*>(buf.p);
+  JC_TRY(jc::launch_fill_u64(reinterpret_cast<uint64_t*>(d), kWords, 0ull, s));
+  JC_TRY(jc::launch_degree_stats(g->n, g->offsets, g->indices, d, s));
+  unsigned long long h[kWords];
+  JC_TRY(hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, s));
+  JC_TRY(hipStreamSynchronize(s));
+  for (int b = 0; b < 32; ++b) {
+    stats->rows[b] = (int64_t)h[b];
+    stats->entries[b] = (int64_t)h[32 + b];
+  }
+  stats->isolated = (int64_t)h[64];
+  stats->long_rows = (int64_t)h[65];
+  stats->long_bitmap_rows[0] = (int64_t)h[66];
+  stats->long_bitmap_rows[1] = (int64_t)h[67];
+  stats->max_degree = (int32_t)(uint32_t)h[68];
+  return JC_OK;
+}
+
+int jc_config_from_stats(const jc_graph_stats* st, int32_t n, int weighted, int* edge_centric,
+                         jc_tuning* tuning) {
+  if (!st || !edge_centric || !tuning || n < 0) return JC_ERR_INVALID_ARG;
+  (void)weighted;  // the measured choices are the same for both
+  *edge_centric = JC_EDGE_CENTRIC;
+  *tuning = jc_tuning{};
+  tuning->small_max = jc::kSmallMax;
+  tuning->small_blocks_per_cu = 32;
+  tuning->bitmap_blocks_per_cu = 1;
+  tuning->hash_table = n >= jc::kQuotMaxN ? 1 : 0;
+  return JC_OK;
+}
+
+int jc_select_config(const jc_csr* g, int weighted, int* edge_centric, jc_tuning* tuning, jc_stream_t stream) {
+  if (!csr_args_ok(g) || !edge_centric || !tuning) return JC_ERR_INVALID_ARG;
+  jc_graph_stats st;
+  const int rc = jc_degree_stats(g, &st, stream);
+  if (rc != JC_OK) return rc;
+  return jc_config_from_stats(&st, g->n, weighted, edge_centric, tuning);
+}
+
 int jc_validate_csr(const jc_csr* g, int* what, jc_stream_t stream) {
   if (what) *what = 0;
   if (!csr_args_ok(g)) return JC_ERR_INVALID_ARG;

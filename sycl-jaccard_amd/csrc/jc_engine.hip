@@ -34,6 +34,61 @@ hipError_t run_engine(int mode, bool weighted, const EngineArgs& base, void* ws,
 }
 
 // ---------------------------------------------------------------------------
+// Degree statistics (jc_degree_stats): one thread per row, counts kept in LDS
+// per workgroup and added to the global totals once. Layout of out: rows[32],
+// entries[32], isolated, long rows, long rows whose bitmap fits (unweighted,
+// weighted table layout), then the largest degree as the low word of out[68].
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_degree_stats(int32_t n, const int32_t* __restrict__ off,
+                                                        const int32_t* __restrict__ ind, size_t bud_u,
+                                                        size_t bud_w, unsigned long long* out) {
+  __shared__ unsigned long long s[68];
+  __shared__ int s_max;
+  for (int i = threadIdx.x; i < 68; i += BLOCK) s[i] = 0ull;
+  if (threadIdx.x == 0) s_max = 0;
+  __syncthreads();
+  int dmax = 0;
+  for (int32_t v = blockIdx.x * BLOCK + threadIdx.x; v < n; v += gridDim.x * BLOCK) {
+    const int o = off[v], d = off[v + 1] - o;
+    if (d <= 0) {
+      atomicAdd(&s[64], 1ull);
+      continue;
+    }
+    const int b = 31 - __clz(d);
+    atomicAdd(&s[b], 1ull);
+    atomicAdd(&s[32 + b], (unsigned long long)d);
+    dmax = max(dmax, d);
+    if (d > kSmallMax) {
+      atomicAdd(&s[65], 1ull);
+      const int32_t lo = ind[o], hi = ind[o + d - 1];
+      if (bitmap_fits<false>(lo, hi, d, bud_u)) atomicAdd(&s[66], 1ull);
+      if (bitmap_fits<true>(lo, hi, d, bud_w)) atomicAdd(&s[67], 1ull);
+    }
+  }
+  atomicMax(&s_max, dmax);
+  __syncthreads();
+  for (int i = threadIdx.x; i < 68; i += BLOCK)
+    if (s[i]) atomicAdd(&out[i], s[i]);
+  if (threadIdx.x == 0) atomicMax(reinterpret_cast<int*>(&out[68]), s_max);
+}
+
+void bitmap_table_bytes(int64_t out[2]) {
+  out[0] = (int64_t)BlockCfg<false>::bm_budget1;
+  out[1] = (int64_t)BlockCfg<true>::bm_budget1;
+}
+
+hipError_t launch_degree_stats(int32_t n, const int32_t* off, const int32_t* ind, unsigned long long* out,
+                               hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  constexpr int B = 256;
+  // the bitmap budgets of the workgroup kernel with one workgroup per CU
+  const size_t bud_u = BlockCfg<false>::bm_budget1, bud_w = BlockCfg<true>::bm_budget1;
+  const int g = (int)std::min<int64_t>((n + B - 1) / B, (int64_t)device_cu_count() * 8);
+  hipLaunchKernelGGL((k_degree_stats<B>), dim3(g), dim3(B), 0, stream, n, off, ind, bud_u, bud_w, out);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Pair list: one wave per pair.
 template <int MODE, bool W, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_pairs(int32_t n, const int32_t* __restrict__ off,

@@ -23,6 +23,10 @@ enum Mode : int {
 #endif
 constexpr int kSmallMax = 128;
 constexpr int kTaskSpan = JC_TASK_SPAN;
+// Graphs with fewer vertices than this use quotient hash tables (16-bit keys):
+// every id is at most 2^24 - 2, so the padding key of a row's last pass
+// (INT32_MAX, whose low 24 bits are 2^24 - 1) hashes like no id.
+constexpr int32_t kQuotMaxN = 1 << 24;
 
 // Range facts of the vertex weights of a weighted pass (launch_exact_info):
 // every weight is a multiple of 2^min_lsb, every row sum is below 2^max_exp,
@@ -134,6 +138,12 @@ hipError_t launch_vertex_weights(int32_t n, const int32_t* off, const float* edg
 hipError_t launch_jw_f32(int64_t m, const float* wi, const float* ws, float* out, hipStream_t s);
 hipError_t launch_jw_f64(int64_t m, const double* wi, const double* ws, float* out, hipStream_t s);
 hipError_t launch_ec_scan(int32_t n, const int32_t* off, int32_t* dest, float* wj, hipStream_t s);
+// bitmap table region of the workgroup kernel with one workgroup per CU:
+// [0] unweighted, [1] weighted layout
+void bitmap_table_bytes(int64_t out[2]);
+// jc_degree_stats: 69 words (see k_degree_stats), zeroed by the caller
+hipError_t launch_degree_stats(int32_t n, const int32_t* off, const int32_t* ind, unsigned long long* out,
+                               hipStream_t s);
 hipError_t launch_count_above(int64_t m, const float* x, float thresh, unsigned long long* cnt,
                               hipStream_t s);
 hipError_t launch_validate(int32_t n, int64_t m, const int32_t* off, const int32_t* ind,

@@ -113,6 +113,9 @@ def _declare_jaccard(lib: ctypes.CDLL) -> None:
         "jc_validate_csr": [vp, ctypes.POINTER(I), vp],
         "jc_csr_is_symmetric": [vp, ctypes.POINTER(I), vp],
         "jc_weighted_sum_mode": [vp, vp, ctypes.POINTER(I), vp],
+        "jc_degree_stats": [vp, vp, vp],
+        "jc_config_from_stats": [vp, i32, I, ctypes.POINTER(I), vp],
+        "jc_select_config": [vp, I, ctypes.POINTER(I), vp, vp],
         "jc_csr_from_edges": [i32, i64, vp, vp, vp, I, vp, vp, vp, i64, ctypes.POINTER(i64), vp],
         "jc_profile_enable": [I],
         "jc_profile_reset": [],

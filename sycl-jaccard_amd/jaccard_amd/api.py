@@ -386,6 +386,50 @@ def weighted_sum_mode(graph: DeviceCSR, weights, stream=None) -> int:
     return int(mode.value)
 
 
+class JcGraphStats(ctypes.Structure):
+    """``jc_graph_stats``: degree statistics of a graph (jc_degree_stats)."""
+    _fields_ = [("rows", ctypes.c_int64 * 32), ("entries", ctypes.c_int64 * 32), ("isolated", ctypes.c_int64),
+                ("long_rows", ctypes.c_int64), ("long_bitmap_rows", ctypes.c_int64 * 2),
+                ("max_degree", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("bitmap_table_bytes", ctypes.c_int64 * 2)]
+
+    def as_dict(self) -> dict:
+        return {"rows": list(self.rows), "entries": list(self.entries), "isolated": int(self.isolated),
+                "long_rows": int(self.long_rows), "long_bitmap_rows": list(self.long_bitmap_rows),
+                "max_degree": int(self.max_degree), "bitmap_table_bytes": list(self.bitmap_table_bytes)}
+
+
+_TUNING_FIELDS = ("small_max", "small_blocks_per_cu", "wide_index", "bitmap_blocks_per_cu", "hash_table",
+                  "quot_max_disp")
+
+
+def degree_stats(graph: DeviceCSR, stream=None) -> JcGraphStats:
+    """Degree histogram and long-row facts of ``graph`` (jc_degree_stats). Synchronous."""
+    st = JcGraphStats()
+    g = graph.c_struct()
+    _check(load_jaccard_lib().jc_degree_stats(ctypes.byref(g), ctypes.byref(st), _stream(stream)),
+           "jc_degree_stats")
+    return st
+
+
+def config_from_stats(stats: JcGraphStats, n: int, weighted: bool):
+    """(edge_centric, tuning dict) that jc_config_from_stats picks (host only)."""
+    ec, t = ctypes.c_int(-1), JcTuning()
+    _check(load_jaccard_lib().jc_config_from_stats(ctypes.byref(stats), int(n), int(bool(weighted)),
+                                                   ctypes.byref(ec), ctypes.byref(t)), "jc_config_from_stats")
+    return ec.value == EDGE_CENTRIC, {f: int(getattr(t, f)) for f in _TUNING_FIELDS}
+
+
+def select_config(graph: DeviceCSR, weighted: bool, stream=None):
+    """(edge_centric, tuning dict) chosen from ``graph``'s degree statistics
+    (jc_select_config), for ``jaccard(..., edge_centric=..., tuning=...)``. Synchronous."""
+    ec, t = ctypes.c_int(-1), JcTuning()
+    g = graph.c_struct()
+    _check(load_jaccard_lib().jc_select_config(ctypes.byref(g), int(bool(weighted)), ctypes.byref(ec),
+                                               ctypes.byref(t), _stream(stream)), "jc_select_config")
+    return ec.value == EDGE_CENTRIC, {f: int(getattr(t, f)) for f in _TUNING_FIELDS}
+
+
 def is_symmetric(graph: DeviceCSR, stream=None) -> bool:
     lib = load_jaccard_lib()
     sym = ctypes.c_int(0)

@@ -143,13 +143,26 @@ class BatchRunner:
         for it, t in zip(self.items, self.tuning):
             self._run_item(it, t, stream)
 
+    def select(self):
+        """Per batch, the launch parameters jc_select_config picks from the
+        union's degree statistics (no timing runs)."""
+        for i, it in enumerate(self.items):
+            _, t = api.select_config(it["dg"], it["W"] is not None)
+            self.tuning[i] = (t["small_max"], t["small_blocks_per_cu"], t["bitmap_blocks_per_cu"])
+        return list(self.tuning)
+
     def autotune(self, reps: int = 2):
-        """Per batch, the fastest launch parameters of TUNING_SPACE."""
+        """Per batch, the fastest launch parameters of TUNING_SPACE and of the
+        degree-statistics choice (select)."""
         torch = api._torch()
         report = []
+        chosen = self.select()
         for i, it in enumerate(self.items):
             best = None
-            for cand in self.TUNING_SPACE:
+            space = list(self.TUNING_SPACE)
+            if chosen[i] not in space:
+                space.insert(0, chosen[i])
+            for cand in space:
                 self._run_item(it, cand, None)
                 torch.cuda.synchronize()
                 t0 = time.perf_counter()

@@ -1419,6 +1419,7 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
     const bool pre = KIND == 1 && s_pre != 0;
     int32_t u, lo0, hi0;
     int ou, du, ka;
+    int npieces = 1;  // pieces of kTaskSpan entries this task covers
     if (pre) {
       u = s_hdr[0];
       ou = s_hdr[1];
@@ -1431,267 +1432,286 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
       u = tk.x;
       ou = a.off[u];
       du = a.off[u + 1] - ou;
-      ka = ou + tk.y * kTaskSpan;
+      // (u, -1): the whole row (hash-table rows of one chunk); (u, p): piece p
+      ka = ou + max(tk.y, 0) * kTaskSpan;
+      if (KIND != 1 && tk.y < 0) npieces = (du + kTaskSpan - 1) / kTaskSpan;
       lo0 = a.ind[ou];
       hi0 = a.ind[ou + du - 1];
     }
-    const int span = min(kTaskSpan, ou + du - ka);
     // bitmap tasks: the row's id span fits the table region of the LDS
     // (bitmap_fits, decided by the scheduler)
     const uint32_t words = ((uint32_t)(hi0 - lo0) >> 5) + 1u;
     const int nch = KIND == 1 ? 1 : (du + CH - 1) / CH;
+    bool built = true;  // hash-table tasks: false once handed to the 32-bit redo pass
+    for (int pc = 0; pc < npieces && built; ++pc) {
+      if (pc > 0) {
+        ka += kTaskSpan;
+        __syncthreads();  // the previous piece's emits have read the item list
+      }
+      const int span = min(kTaskSpan, ou + du - ka);
 
-    // owned entries of the task, sorted into length classes (heaviest first)
-    int32_t x = 0;
-    int ox = 0, dx = 0, cls = -1;
-    if (tid < span) {
-      if (pre) {
-        x = st_x[tid];
-        ox = st_ox[tid];
-        dx = st_dx[tid];
-      } else {
-        x = a.ind[ka + tid];
-        ox = a.off[x];
-        dx = a.off[x + 1] - ox;
+      // owned entries of the piece, sorted into length classes (heaviest first)
+      int32_t x = 0;
+      int ox = 0, dx = 0, cls = -1;
+      if (tid < span) {
+        if (pre) {
+          x = st_x[tid];
+          ox = st_ox[tid];
+          dx = st_dx[tid];
+        } else {
+          x = a.ind[ka + tid];
+          ox = a.off[x];
+          dx = a.off[x + 1] - ox;
+        }
+        if (owns(u, du, x, dx)) cls = len_class((dx + nch - 1) / nch);
       }
-      if (owns(u, du, x, dx)) cls = len_class((dx + nch - 1) / nch);
-    }
-    int rank = 0;
-#pragma unroll
-    for (int k = 0; k < kClasses; ++k) {
-      const uint64_t bm = ballot(cls == k);
-      if (cls == k) rank = popc64(bm & ((1ull << lane) - 1));
-      if (lane == 0) s_wcnt[k][w] = popc64(bm);
-    }
-    __syncthreads();
-    if (tid == 0) {
-      int ib = 0, ub = 0;
-      for (int k = kClasses - 1; k >= 0; --k) {
-        int c = 0;
-        for (int v = 0; v < NW; ++v) c += s_wcnt[k][v];
-        s_cbase[kClasses - 1 - k] = ib;
-        s_ubase[kClasses - 1 - k] = ub;
-        ib += c;
-        ub += (c + (kWave / class_g(k)) - 1) / (kWave / class_g(k));
+      int rank = 0;
+  #pragma unroll
+      for (int k = 0; k < kClasses; ++k) {
+        const uint64_t bm = ballot(cls == k);
+        if (cls == k) rank = popc64(bm & ((1ull << lane) - 1));
+        if (lane == 0) s_wcnt[k][w] = popc64(bm);
       }
-      s_cbase[kClasses] = ib;
-      s_ubase[kClasses] = ub;
-    }
-    __syncthreads();
-    if (cls >= 0) {
-      int at = s_cbase[kClasses - 1 - cls] + rank;
-      for (int v = 0; v < w; ++v) at += s_wcnt[cls][v];
-      ix[at] = x;
-      iox[at] = ox;
-      idx_[at] = dx;
-      ij[at] = (uint16_t)tid;
-    }
-    const int nunits = s_ubase[kClasses];
-    if constexpr (KIND == 1) {
-      // the words and the zero guard word (weighted: interleaved with
-      // 1 + their ranks), then, weighted, the zero slot and the keys'
-      // weights in id order as fp64
-      constexpr uint32_t WS = W ? 2u : 1u;  // stride of the words
-      uint32_t* bits = reinterpret_cast<uint32_t*>(smem);
-      double* bval = reinterpret_cast<double*>(bits + WS * (words + 1u));
-      // words plus the zero guard word (weighted: with its rank slot)
-      for (uint32_t i = tid; i < WS * (words + 1u); i += BLOCK) bits[i] = 0u;
-      if (tid == 0) s_next = 0;
       __syncthreads();
-      if (du <= kTaskSpan) {
-        // the task is the whole row: its entries are the keys, in id order
-        if (tid < du) {
-          const uint32_t o = (uint32_t)(x - lo0);
-          if (o < words * 32u) atomicOr(&bits[WS * (o >> 5)], 1u << (o & 31u));  // rows are sorted: always true
-          if constexpr (W) bval[1 + tid] = (double)(pre ? st_w[tid] : a.vw[x]);
+      if (tid == 0) {
+        int ib = 0, ub = 0;
+        for (int k = kClasses - 1; k >= 0; --k) {
+          int c = 0;
+          for (int v = 0; v < NW; ++v) c += s_wcnt[k][v];
+          s_cbase[kClasses - 1 - k] = ib;
+          s_ubase[kClasses - 1 - k] = ub;
+          ib += c;
+          ub += (c + (kWave / class_g(k)) - 1) / (kWave / class_g(k));
         }
-      } else {
-        for (int i = tid; i < du; i += BLOCK) {
-          const int32_t y = a.ind[ou + i];
-          const uint32_t o = (uint32_t)(y - lo0);
-          if (o < words * 32u) atomicOr(&bits[WS * (o >> 5)], 1u << (o & 31u));  // rows are sorted: always true
-          if constexpr (W) bval[1 + i] = (double)a.vw[y];  // the i-th key in id order has rank i
-        }
+        s_cbase[kClasses] = ib;
+        s_ubase[kClasses] = ub;
       }
-      if (W && tid == 0) bval[0] = 0.0;
       __syncthreads();
-      if constexpr (W) {
-        const uint32_t per = (words + BLOCK - 1) / BLOCK;
-        const uint32_t w0 = min(words, (uint32_t)tid * per), w1 = min(words, w0 + per);
-        int c = 0;
-        for (uint32_t i = w0; i < w1; ++i) c += __popc(bits[2 * i]);
-        int r = block_excl_scan(c, s_wsum);
-        for (uint32_t i = w0; i < w1; ++i) {
-          bits[2 * i + 1] = (uint32_t)r + 1u;
-          r += __popc(bits[2 * i]);
-        }
+      if (cls >= 0) {
+        int at = s_cbase[kClasses - 1 - cls] + rank;
+        for (int v = 0; v < w; ++v) at += s_wcnt[cls][v];
+        ix[at] = x;
+        iox[at] = ox;
+        idx_[at] = dx;
+        ij[at] = (uint16_t)tid;
+      }
+      const int nunits = s_ubase[kClasses];
+      if constexpr (KIND == 1) {
+        // the words and the zero guard word (weighted: interleaved with
+        // 1 + their ranks), then, weighted, the zero slot and the keys'
+        // weights in id order as fp64
+        constexpr uint32_t WS = W ? 2u : 1u;  // stride of the words
+        uint32_t* bits = reinterpret_cast<uint32_t*>(smem);
+        double* bval = reinterpret_cast<double*>(bits + WS * (words + 1u));
+        // words plus the zero guard word (weighted: with its rank slot)
+        for (uint32_t i = tid; i < WS * (words + 1u); i += BLOCK) bits[i] = 0u;
+        if (tid == 0) s_next = 0;
         __syncthreads();
-      }
-      Chunk<W> cx;
-      cx.u = u;
-      cx.du = du;
-      cx.su = row_sum_of<MODE>(a, u);
-      cx.lo = lo0;
-      cx.hi = hi0;
-      cx.c = 0;
-      cx.nch = 1;
-      cx.kbase = ka;
-      cx.cursor = cursor;
-      cx.pcnt = pcnt;
-      cx.pacc = pacc;
-      cx.bits = bits;
-      cx.bval = bval;
-      cx.wzero = words;
-      if (w == 0) {
-        // lookahead (wave 0): reserve the next task and stage it; the loads
-        // of all its entries are issued together (three round trips)
-        int tn = 0;
-        if (lane == 0) tn = atomicAdd(counter, 1);
-        tn = __shfl(tn, 0);
-        if (tn < ntasks) {
-          const int2 tk2 = a.tasks[tbase + tn];
-          const int ou2 = a.off[tk2.x];
-          const int du2 = a.off[tk2.x + 1] - ou2;
-          const int ka2 = ou2 + tk2.y * kTaskSpan;
-          const int span2 = min(kTaskSpan, ou2 + du2 - ka2);
-          const int32_t lo2 = a.ind[ou2], hi2 = a.ind[ou2 + du2 - 1];
-          // loads are unconditional, their index clamped to the task's last
-          // entry (span2 >= 1), so none waits behind a branch; only the
-          // stores are limited to the task
-          constexpr int NI = (kTaskSpan + kWave - 1) / kWave;
-          int32_t xs[NI];
-#pragma unroll
-          for (int k = 0; k < NI; ++k) xs[k] = a.ind[ka2 + min(lane + k * kWave, span2 - 1)];
-          int o0[NI], o1[NI];
-#pragma unroll
-          for (int k = 0; k < NI; ++k) {
-            o0[k] = a.off[xs[k]];
-            o1[k] = a.off[xs[k] + 1];
-          }
-          if constexpr (W) {
-            if (du2 <= kTaskSpan) {
-              float ws[NI];
-#pragma unroll
-              for (int k = 0; k < NI; ++k) ws[k] = a.vw[xs[k]];
-#pragma unroll
-              for (int k = 0; k < NI; ++k)
-                if (lane + k * kWave < span2) st_w[lane + k * kWave] = ws[k];
-            }
-          }
-#pragma unroll
-          for (int k = 0; k < NI; ++k) {
-            const int i = lane + k * kWave;
-            if (i < span2) {
-              st_x[i] = xs[k];
-              st_ox[i] = o0[k];
-              st_dx[i] = o1[k] - o0[k];
-            }
-          }
-          if (lane == 0) {
-            s_hdr[0] = tk2.x;
-            s_hdr[1] = ou2;
-            s_hdr[2] = du2;
-            s_hdr[3] = ka2;
-            s_hdr[4] = lo2;
-            s_hdr[5] = hi2;
-          }
-        }
-        if (lane == 0) {
-          s_task = tn;
-          s_pre = tn < ntasks ? 1 : 0;
-        }
-      }
-      for (;;) {
-        int un = 0;
-        if (lane == 0) un = atomicAdd(&s_next, 1);
-        un = __shfl(un, 0);
-        if (un >= nunits) break;
-        int k = 0;
-        while (un >= s_ubase[k + 1]) ++k;
-        const int cls_k = kClasses - 1 - k;
-        const int per = kWave / class_g(cls_k);
-        const int first = s_cbase[k] + (un - s_ubase[k]) * per;
-        run_unit_lds<MODE, W, false, kTkBitmap>(a, cx, cls_k, first, s_cbase[k + 1], ix, iox, idx_, ij, fast);
-      }
-      __syncthreads();
-      for (int i = tid; i < s_cbase[kClasses]; i += BLOCK)
-        emit_deferred<MODE, W>(a, u, du, cx.su, ka, i, ix[i], idx_[i], ij[i], cursor, pcnt, pacc);
-      continue;
-    }
-    if constexpr (KIND != 1) {
-      constexpr int TK = kQ ? kTkQuot : kTkHash;
-      bool built = true;  // false: the task was handed to the 32-bit redo pass
-      for (int c = 0; c < nch; ++c) {
-        const int cs = ou + c * CH;
-        const int ce = min(cs + CH, ou + du);
-        const int clen = ce - cs;
-        Chunk<W> cx;
-        if constexpr (kQ) {
-          const int lb = qtable_log_buckets(clen);
-          const int maxd = (1 << (lb - 8)) - 2;  // 2^(16 - rb) - 2
-          cx.qtab = QTable<W>{qword, qval, lb, (uint32_t)((1 << lb) - 1), 24 - lb,
-                              a.quot_max_disp > 0 ? min(maxd, a.quot_max_disp - 1) : maxd};
-          for (int i = tid; i < (1 << lb); i += BLOCK) cx.qtab.clear(i);
-          if (tid == 0) {
-            s_next = 0;
-            s_fail = 0;
-          }
-          __syncthreads();
-          build_qtable<W, BLOCK, KPT>(cx.qtab, a.ind, a.vw, cs, clen, tid, &s_fail, s_wsum);
-          __syncthreads();
-          if (s_fail) {
-            // nothing of this task has been written yet: the 32-bit table redoes it
-            if (tid == 0) a.redo[atomicAdd(a.num_redo, 1)] = t;
-            built = false;
-            break;
+        if (du <= kTaskSpan) {
+          // the task is the whole row: its entries are the keys, in id order
+          if (tid < du) {
+            const uint32_t o = (uint32_t)(x - lo0);
+            if (o < words * 32u) atomicOr(&bits[WS * (o >> 5)], 1u << (o & 31u));  // rows are sorted: always true
+            if constexpr (W) bval[1 + tid] = (double)(pre ? st_w[tid] : a.vw[x]);
           }
         } else {
-          const int lb = table_log_buckets<W>(clen);
-          cx.tab = Table<W>{tkey, tval, lb, (uint32_t)((1 << lb) - 1), Cfg::NB * 4};
-          for (int i = tid; i < (4 << lb); i += BLOCK) cx.tab.clear(i);
-          if (tid == 0) {
-            s_next = 0;
-            cx.tab.set_zero();
+          for (int i = tid; i < du; i += BLOCK) {
+            const int32_t y = a.ind[ou + i];
+            const uint32_t o = (uint32_t)(y - lo0);
+            if (o < words * 32u) atomicOr(&bits[WS * (o >> 5)], 1u << (o & 31u));  // rows are sorted: always true
+            if constexpr (W) bval[1 + i] = (double)a.vw[y];  // the i-th key in id order has rank i
+          }
+        }
+        if (W && tid == 0) bval[0] = 0.0;
+        __syncthreads();
+        if constexpr (W) {
+          const uint32_t per = (words + BLOCK - 1) / BLOCK;
+          const uint32_t w0 = min(words, (uint32_t)tid * per), w1 = min(words, w0 + per);
+          int c = 0;
+          for (uint32_t i = w0; i < w1; ++i) c += __popc(bits[2 * i]);
+          int r = block_excl_scan(c, s_wsum);
+          for (uint32_t i = w0; i < w1; ++i) {
+            bits[2 * i + 1] = (uint32_t)r + 1u;
+            r += __popc(bits[2 * i]);
           }
           __syncthreads();
-          build_table<W>(cx.tab, a.ind, a.vw, cs, clen, tid, BLOCK);
-          __syncthreads();
         }
+        Chunk<W> cx;
         cx.u = u;
         cx.du = du;
         cx.su = row_sum_of<MODE>(a, u);
-        cx.lo = a.ind[cs];
-        cx.hi = a.ind[ce - 1];
-        cx.c = c;
-        cx.nch = nch;
+        cx.lo = lo0;
+        cx.hi = hi0;
+        cx.c = 0;
+        cx.nch = 1;
         cx.kbase = ka;
         cx.cursor = cursor;
         cx.pcnt = pcnt;
         cx.pacc = pacc;
+        cx.bits = bits;
+        cx.bval = bval;
+        cx.wzero = words;
+        if (w == 0) {
+          // lookahead (wave 0): reserve the next task and stage it; the loads
+          // of all its entries are issued together (three round trips)
+          int tn = 0;
+          if (lane == 0) tn = atomicAdd(counter, 1);
+          tn = __shfl(tn, 0);
+          if (tn < ntasks) {
+            const int2 tk2 = a.tasks[tbase + tn];
+            const int ou2 = a.off[tk2.x];
+            const int du2 = a.off[tk2.x + 1] - ou2;
+            const int ka2 = ou2 + tk2.y * kTaskSpan;
+            const int span2 = min(kTaskSpan, ou2 + du2 - ka2);
+            const int32_t lo2 = a.ind[ou2], hi2 = a.ind[ou2 + du2 - 1];
+            // loads are unconditional, their index clamped to the task's last
+            // entry (span2 >= 1), so none waits behind a branch; only the
+            // stores are limited to the task
+            constexpr int NI = (kTaskSpan + kWave - 1) / kWave;
+            int32_t xs[NI];
+  #pragma unroll
+            for (int k = 0; k < NI; ++k) xs[k] = a.ind[ka2 + min(lane + k * kWave, span2 - 1)];
+            int o0[NI], o1[NI];
+  #pragma unroll
+            for (int k = 0; k < NI; ++k) {
+              o0[k] = a.off[xs[k]];
+              o1[k] = a.off[xs[k] + 1];
+            }
+            if constexpr (W) {
+              if (du2 <= kTaskSpan) {
+                float ws[NI];
+  #pragma unroll
+                for (int k = 0; k < NI; ++k) ws[k] = a.vw[xs[k]];
+  #pragma unroll
+                for (int k = 0; k < NI; ++k)
+                  if (lane + k * kWave < span2) st_w[lane + k * kWave] = ws[k];
+              }
+            }
+  #pragma unroll
+            for (int k = 0; k < NI; ++k) {
+              const int i = lane + k * kWave;
+              if (i < span2) {
+                st_x[i] = xs[k];
+                st_ox[i] = o0[k];
+                st_dx[i] = o1[k] - o0[k];
+              }
+            }
+            if (lane == 0) {
+              s_hdr[0] = tk2.x;
+              s_hdr[1] = ou2;
+              s_hdr[2] = du2;
+              s_hdr[3] = ka2;
+              s_hdr[4] = lo2;
+              s_hdr[5] = hi2;
+            }
+          }
+          if (lane == 0) {
+            s_task = tn;
+            s_pre = tn < ntasks ? 1 : 0;
+          }
+        }
         for (;;) {
           int un = 0;
           if (lane == 0) un = atomicAdd(&s_next, 1);
           un = __shfl(un, 0);
           if (un >= nunits) break;
           int k = 0;
-          while (un >= s_ubase[k + 1]) ++k;  // class slot k (0 = heaviest)
+          while (un >= s_ubase[k + 1]) ++k;
           const int cls_k = kClasses - 1 - k;
           const int per = kWave / class_g(cls_k);
           const int first = s_cbase[k] + (un - s_ubase[k]) * per;
-          const int cend = s_cbase[k + 1];
-          if (nch == 1)
-            run_unit_lds<MODE, W, false, TK>(a, cx, cls_k, first, cend, ix, iox, idx_, ij, fast);
-          else
-            run_unit_lds<MODE, W, true, TK>(a, cx, cls_k, first, cend, ix, iox, idx_, ij, fast);
+          run_unit_lds<MODE, W, false, kTkBitmap>(a, cx, cls_k, first, s_cbase[k + 1], ix, iox, idx_, ij, fast);
         }
         __syncthreads();
-      }
-      if (built) {
-        const double su = row_sum_of<MODE>(a, u);
         for (int i = tid; i < s_cbase[kClasses]; i += BLOCK)
-          emit_deferred<MODE, W>(a, u, du, su, ka, i, ix[i], idx_[i], ij[i], cursor, pcnt, pacc);
+          emit_deferred<MODE, W>(a, u, du, cx.su, ka, i, ix[i], idx_[i], ij[i], cursor, pcnt, pacc);
+        continue;
       }
-    }
+      if constexpr (KIND != 1) {
+        constexpr int TK = kQ ? kTkQuot : kTkHash;
+        for (int c = 0; c < nch; ++c) {
+          const int cs = ou + c * CH;
+          const int ce = min(cs + CH, ou + du);
+          const int clen = ce - cs;
+          // a single-chunk table stays in the LDS for the task's later pieces
+          const bool fresh = nch > 1 || pc == 0;
+          Chunk<W> cx;
+          if constexpr (kQ) {
+            const int lb = qtable_log_buckets(clen);
+            const int maxd = (1 << (lb - 8)) - 2;  // 2^(16 - rb) - 2
+            cx.qtab = QTable<W>{qword, qval, lb, (uint32_t)((1 << lb) - 1), 24 - lb,
+                                a.quot_max_disp > 0 ? min(maxd, a.quot_max_disp - 1) : maxd};
+            if (fresh) {
+              for (int i = tid; i < (1 << lb); i += BLOCK) cx.qtab.clear(i);
+              if (tid == 0) {
+                s_next = 0;
+                s_fail = 0;
+              }
+              __syncthreads();
+              build_qtable<W, BLOCK, KPT>(cx.qtab, a.ind, a.vw, cs, clen, tid, &s_fail, s_wsum);
+              __syncthreads();
+              if (s_fail) {
+                // nothing of this task has been written yet (a failed build is
+                // the first of its pieces): the 32-bit table redoes it
+                if (tid == 0) a.redo[atomicAdd(a.num_redo, 1)] = t;
+                built = false;
+                break;
+              }
+            }
+          } else {
+            const int lb = table_log_buckets<W>(clen);
+            cx.tab = Table<W>{tkey, tval, lb, (uint32_t)((1 << lb) - 1), Cfg::NB * 4};
+            if (fresh) {
+              for (int i = tid; i < (4 << lb); i += BLOCK) cx.tab.clear(i);
+              if (tid == 0) {
+                s_next = 0;
+                cx.tab.set_zero();
+              }
+              __syncthreads();
+              build_table<W>(cx.tab, a.ind, a.vw, cs, clen, tid, BLOCK);
+              __syncthreads();
+            }
+          }
+          if (!fresh) {
+            if (tid == 0) s_next = 0;
+            __syncthreads();  // the piece's item list is complete
+          }
+          cx.u = u;
+          cx.du = du;
+          cx.su = row_sum_of<MODE>(a, u);
+          cx.lo = a.ind[cs];
+          cx.hi = a.ind[ce - 1];
+          cx.c = c;
+          cx.nch = nch;
+          cx.kbase = ka;
+          cx.cursor = cursor;
+          cx.pcnt = pcnt;
+          cx.pacc = pacc;
+          for (;;) {
+            int un = 0;
+            if (lane == 0) un = atomicAdd(&s_next, 1);
+            un = __shfl(un, 0);
+            if (un >= nunits) break;
+            int k = 0;
+            while (un >= s_ubase[k + 1]) ++k;  // class slot k (0 = heaviest)
+            const int cls_k = kClasses - 1 - k;
+            const int per = kWave / class_g(cls_k);
+            const int first = s_cbase[k] + (un - s_ubase[k]) * per;
+            const int cend = s_cbase[k + 1];
+            if (nch == 1)
+              run_unit_lds<MODE, W, false, TK>(a, cx, cls_k, first, cend, ix, iox, idx_, ij, fast);
+            else
+              run_unit_lds<MODE, W, true, TK>(a, cx, cls_k, first, cend, ix, iox, idx_, ij, fast);
+          }
+          __syncthreads();
+        }
+        if (built) {
+          const double su = row_sum_of<MODE>(a, u);
+          for (int i = tid; i < s_cbase[kClasses]; i += BLOCK)
+            emit_deferred<MODE, W>(a, u, du, su, ka, i, ix[i], idx_[i], ij[i], cursor, pcnt, pacc);
+        }
+      }
+    }  // pieces
   }
 }
 
@@ -1816,10 +1836,19 @@ __device__ __forceinline__ int32_t seg_row(const int2* __restrict__ segs, int ns
   return segs[lo].x + (v - segs[lo].y);
 }
 
+// Tasks of a row of d > small_max entries of table kind `kind`: a hash-table
+// row whose keys fit one table chunk (d <= whole_max) is one task covering
+// all its pieces of kTaskSpan entries, so its table is built once; other
+// rows get one task per piece.
+__device__ __forceinline__ int row_tasks(int d, int kind, int whole_max) {
+  return (kind == 0 && d <= whole_max) ? 1 : (d + kTaskSpan - 1) / kTaskSpan;
+}
+
 template <bool W, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_sched_count(int32_t rows, const int32_t* __restrict__ off,
                                                        const int32_t* __restrict__ ind, Sched* sc, int small_max,
-                                                       size_t bm_budget, const int2* segs, int nseg) {
+                                                       size_t bm_budget, int whole_max, const int2* segs,
+                                                       int nseg) {
   __shared__ int hist[2 * kBuckets + 1];
   for (int i = threadIdx.x; i < 2 * kBuckets + 1; i += BLOCK) hist[i] = 0;
   __syncthreads();
@@ -1831,7 +1860,7 @@ __global__ __launch_bounds__(BLOCK) void k_sched_count(int32_t rows, const int32
       atomicAdd(&hist[2 * kBuckets], 1);
     } else {
       const int kind = bitmap_fits<W>(ind[o], ind[o + d - 1], d, bm_budget) ? 1 : 0;
-      atomicAdd(&hist[kind * kBuckets + bucket_of(d)], (d + kTaskSpan - 1) / kTaskSpan);
+      atomicAdd(&hist[kind * kBuckets + bucket_of(d)], row_tasks(d, kind, whole_max));
     }
   }
   __syncthreads();
@@ -1864,7 +1893,7 @@ template <bool W, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_sched_emit(int32_t rows, const int32_t* __restrict__ off,
                                                       const int32_t* __restrict__ ind, Sched* sc, int2* tasks,
                                                       int32_t* small_rows, int small_max, size_t bm_budget,
-                                                      const int2* segs, int nseg) {
+                                                      int whole_max, const int2* segs, int nseg) {
   constexpr int kSmallBin = 2 * kBuckets;
   __shared__ int hist[kSmallBin + 1];
   __shared__ int basep[kSmallBin + 1];
@@ -1884,7 +1913,7 @@ __global__ __launch_bounds__(BLOCK) void k_sched_emit(int32_t rows, const int32_
         } else {
           const int kind = bitmap_fits<W>(ind[o], ind[o + d - 1], d, bm_budget) ? 1 : 0;
           b = kind * kBuckets + bucket_of(d);
-          cntv = (d + kTaskSpan - 1) / kTaskSpan;
+          cntv = row_tasks(d, kind, whole_max);
         }
         local = atomicAdd(&hist[b], cntv);
       }
@@ -1900,7 +1929,11 @@ __global__ __launch_bounds__(BLOCK) void k_sched_emit(int32_t rows, const int32_
       small_rows[basep[b] + local] = u;
     } else if (b >= 0) {
       const int at = basep[b] + local;
-      for (int p = 0; p < cntv; ++p) tasks[at + p] = make_int2(u, p);
+      // a whole-row task is (u, -1); a piece task (u, piece)
+      if (b < kBuckets && cntv == 1 && d > kTaskSpan)
+        tasks[at] = make_int2(u, -1);
+      else
+        for (int p = 0; p < cntv; ++p) tasks[at + p] = make_int2(u, p);
     }
     __syncthreads();
   }
@@ -1962,13 +1995,15 @@ hipError_t run_engine_t(const EngineArgs& base, void* ws, bool fixup, hipStream_
     const size_t bm_budget = bm_per_cu == 2 ? kBitmapLds2 : Cfg::bm_budget1;
     const size_t lds1 = bm_budget + bm_rest;
     a.bm_budget = (uint32_t)bm_budget;
+    // hash-table rows up to one chunk of keys are whole-row tasks
+    const int whole_max = quot ? Cfg::QCH : Cfg::CH;
     {
       ProfScope ps("sched", stream);
       hipLaunchKernelGGL((k_sched_count<W, PB>), dim3(g), dim3(PB), 0, stream, rows, a.off, a.ind, sc, a.small_max,
-                         bm_budget, a.segs, a.nseg);
+                         bm_budget, whole_max, a.segs, a.nseg);
       hipLaunchKernelGGL(k_sched_scan, dim3(1), dim3(64), 0, stream, sc);
       hipLaunchKernelGGL((k_sched_emit<W, PB>), dim3(g), dim3(PB), 0, stream, rows, a.off, a.ind, sc, tasks, small,
-                         a.small_max, bm_budget, a.segs, a.nseg);
+                         a.small_max, bm_budget, whole_max, a.segs, a.nseg);
       if ((err = hipGetLastError()) != hipSuccess) return err;
     }
 

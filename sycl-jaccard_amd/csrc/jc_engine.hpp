@@ -1224,14 +1224,18 @@ struct BlockCfg {
   static constexpr size_t bm_budget1 = kBlockLds - bm_rest;
   // quotient tables: 16 bytes per bucket, weighted also kQLambda dense
   // weights per bucket and the zero slot
-  static constexpr size_t q_avail = kBlockLds - item_bytes - cursor_bytes - part_bytes;
+  // quotient-table tasks also stage the next piece's entries (column,
+  // offset, degree), which leaves the bucket count unchanged
+  static constexpr size_t qstage_bytes = (size_t)kTaskSpan * 3 * sizeof(int32_t);
+  static constexpr size_t q_avail = kBlockLds - item_bytes - qstage_bytes - cursor_bytes - part_bytes;
   static constexpr size_t q_bucket_cost = 16 + (W ? 4 * kQLambda : 0);
   static constexpr int QNB_FIT = (int)((q_avail - 16) / q_bucket_cost);
   static constexpr int QNB = 1 << (31 - __builtin_clz(QNB_FIT));  // power of two
   static constexpr int QCH = kQLambda * QNB;                      // keys per chunk
   static constexpr size_t qword_bytes = (size_t)QNB * 16;
   static constexpr size_t qval_bytes = W ? ((size_t)(QCH + 1) * sizeof(float) + 15) / 16 * 16 : 0;
-  static constexpr size_t q_lds_bytes = qword_bytes + qval_bytes + item_bytes + cursor_bytes + part_bytes;
+  static constexpr size_t q_lds_bytes =
+      qword_bytes + qval_bytes + item_bytes + qstage_bytes + cursor_bytes + part_bytes;
   static_assert(QNB <= (1 << 14) && QCH < (1 << 15), "quotient table: lb <= 14, weight bases fit 15 bits");
   static_assert(QNB >= (1 << kQMinLb), "quotient table smaller than its minimum");
 };
@@ -1373,23 +1377,27 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
   // multi-chunk cursors and partials (hash-table tasks), which also hold the
   // units' results until the task's emits (all tasks); bitmap tasks keep them
   // after their staging area
-  unsigned char* resp = items + Cfg::item_bytes + (KIND == 1 ? Cfg::stage_bytes : 0);
+  unsigned char* resp =
+      items + Cfg::item_bytes + (KIND == 1 ? Cfg::stage_bytes : kQ ? Cfg::qstage_bytes : 0);
   int* cursor = reinterpret_cast<int*>(resp);
   unsigned char* partp = resp + Cfg::cursor_bytes;
   uint32_t* pcnt = reinterpret_cast<uint32_t*>(partp);
   DD* pacc = reinterpret_cast<DD*>(partp);
-  // Bitmap tasks look one task ahead: while the other waves run a task's
-  // units, wave 0 reserves the next task and stages its header and entries
-  // (column, offset, degree; weighted whole-row tasks also the columns'
-  // vertex weights) in LDS after the item list, so a task starts without a
+  // Bitmap and quotient-table tasks look ahead: while the other waves run a
+  // piece's units, wave 0 stages the entries of the next piece (column,
+  // offset, degree; weighted whole-row bitmap tasks also the columns' vertex
+  // weights) in LDS after the item list, and at a task's last piece reserves
+  // the next task and stages its header too, so a piece starts without a
   // global round trip.
-  int32_t* st_x = KIND == 1 ? reinterpret_cast<int32_t*>(items + Cfg::item_bytes) : nullptr;
-  int32_t* st_ox = KIND == 1 ? st_x + kTaskSpan : nullptr;
-  int32_t* st_dx = KIND == 1 ? st_x + 2 * kTaskSpan : nullptr;
+  constexpr bool kLook = KIND != 0;
+  int32_t* st_x = kLook ? reinterpret_cast<int32_t*>(items + Cfg::item_bytes) : nullptr;
+  int32_t* st_ox = kLook ? st_x + kTaskSpan : nullptr;
+  int32_t* st_dx = kLook ? st_x + 2 * kTaskSpan : nullptr;
   float* st_w = KIND == 1 && W ? reinterpret_cast<float*>(st_x + 3 * kTaskSpan) : nullptr;
   __shared__ int s_task;
-  __shared__ int s_pre;      // bitmap tasks: s_task's header and entries are staged
-  __shared__ int s_hdr[6];   // staged header: u, ou, du, ka, first and last column of u
+  __shared__ int s_pre;      // the next piece's entries (at a task's start: and header) are staged
+  __shared__ int s_rsv;      // quotient tasks: s_task was reserved by the lookahead
+  __shared__ int s_hdr[7];   // staged header: u, ou, du, ka, first and last column of u, pieces
   __shared__ int s_next;
   __shared__ int s_wcnt[kClasses][NW];
   __shared__ int s_cbase[kClasses + 1];   // item ranges per class, heaviest first
@@ -1404,19 +1412,27 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
   const int tbase = KIND == 1 ? a.num_tasks[0] : 0;
   int* const counter = &a.task_counter[KIND == 1 ? 1 : redo ? 2 : 0];
   const bool fast = exact_f64(a);
-  if (KIND == 1 && tid == 0) {
+  if (kLook && tid == 0) {
     s_task = atomicAdd(counter, 1);
     s_pre = 0;
+    s_rsv = 1;
   }
   for (;;) {
     __syncthreads();
     if constexpr (KIND != 1) {
-      if (tid == 0) s_task = atomicAdd(counter, 1);
+      if (tid == 0) {
+        // a quotient task that failed before its lookahead ran reserved nothing
+        if (!kQ || !s_rsv) {
+          s_task = atomicAdd(counter, 1);
+          if (kQ) s_pre = 0;
+        }
+        if (kQ) s_rsv = 0;
+      }
       __syncthreads();
     }
     const int t = s_task;
     if (t >= ntasks) return;
-    const bool pre = KIND == 1 && s_pre != 0;
+    const bool pre = kLook && s_pre != 0;
     int32_t u, lo0, hi0;
     int ou, du, ka;
     int npieces = 1;  // pieces of kTaskSpan entries this task covers
@@ -1427,6 +1443,7 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
       ka = s_hdr[3];
       lo0 = s_hdr[4];
       hi0 = s_hdr[5];
+      npieces = kQ ? s_hdr[6] : 1;  // bitmap tasks are one piece
     } else {
       const int2 tk = a.tasks[tbase + (redo ? a.redo[t] : t)];
       u = tk.x;
@@ -1435,8 +1452,9 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
       // (u, -1): the whole row (hash-table rows of one chunk); (u, p): piece p
       ka = ou + max(tk.y, 0) * kTaskSpan;
       if (KIND != 1 && tk.y < 0) npieces = (du + kTaskSpan - 1) / kTaskSpan;
-      lo0 = a.ind[ou];
-      hi0 = a.ind[ou + du - 1];
+      // the row's first and last column: bitmap tasks only
+      lo0 = KIND == 1 ? a.ind[ou] : 0;
+      hi0 = KIND == 1 ? a.ind[ou + du - 1] : 0;
     }
     // bitmap tasks: the row's id span fits the table region of the LDS
     // (bitmap_fits, decided by the scheduler)
@@ -1449,12 +1467,13 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
         __syncthreads();  // the previous piece's emits have read the item list
       }
       const int span = min(kTaskSpan, ou + du - ka);
+      const bool pre_p = pc == 0 ? pre : (kQ && s_pre != 0);
 
       // owned entries of the piece, sorted into length classes (heaviest first)
       int32_t x = 0;
       int ox = 0, dx = 0, cls = -1;
       if (tid < span) {
-        if (pre) {
+        if (pre_p) {
           x = st_x[tid];
           ox = st_ox[tid];
           dx = st_dx[tid];
@@ -1474,6 +1493,7 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
       }
       __syncthreads();
       if (tid == 0) {
+        if (kQ) s_pre = 0;  // every thread has read the staged piece
         int ib = 0, ub = 0;
         for (int k = kClasses - 1; k >= 0; --k) {
           int c = 0;
@@ -1603,6 +1623,7 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
               s_hdr[3] = ka2;
               s_hdr[4] = lo2;
               s_hdr[5] = hi2;
+              s_hdr[6] = 1;
             }
           }
           if (lane == 0) {
@@ -1687,6 +1708,73 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
           cx.cursor = cursor;
           cx.pcnt = pcnt;
           cx.pacc = pacc;
+          if constexpr (kQ) {
+            if (c == 0 && w == 0) {
+              // lookahead (wave 0, then it joins the units): the next piece of
+              // this task, or at its last piece the next task's header and
+              // first piece; loads unconditional with indices clamped to the
+              // piece (span >= 1), stores limited to it
+              const bool same = pc + 1 < npieces;
+              int tn = t, u2 = u, ou2 = ou, du2 = du, ka2 = ka + kTaskSpan, np2 = npieces;
+              if (!same) {
+                int r = 0;
+                if (lane == 0) r = atomicAdd(counter, 1);
+                tn = __shfl(r, 0);
+                if (tn < ntasks) {
+                  const int2 tk2 = a.tasks[tn];
+                  u2 = tk2.x;
+                  ou2 = a.off[u2];
+                  du2 = a.off[u2 + 1] - ou2;
+                  ka2 = ou2 + max(tk2.y, 0) * kTaskSpan;
+                  np2 = tk2.y < 0 ? (du2 + kTaskSpan - 1) / kTaskSpan : 1;
+                }
+              }
+              const bool staged = same || tn < ntasks;
+              if (staged) {
+                // in groups of four entries per lane: twelve live registers,
+                // so the kernel's allocation (and the probe loops') does not
+                // grow into spills
+                const int span2 = min(kTaskSpan, ou2 + du2 - ka2);
+                constexpr int NI = (kTaskSpan + kWave - 1) / kWave;
+                constexpr int GI = 4;
+                for (int k0 = 0; k0 < NI; k0 += GI) {
+                  if (k0 * kWave >= span2) break;  // wave-uniform
+                  int32_t xs[GI];
+                  int o0[GI], o1[GI];
+#pragma unroll
+                  for (int k = 0; k < GI; ++k) xs[k] = a.ind[ka2 + min(lane + (k0 + k) * kWave, span2 - 1)];
+#pragma unroll
+                  for (int k = 0; k < GI; ++k) {
+                    o0[k] = a.off[xs[k]];
+                    o1[k] = a.off[xs[k] + 1];
+                  }
+#pragma unroll
+                  for (int k = 0; k < GI; ++k) {
+                    const int i = lane + (k0 + k) * kWave;
+                    if (i < span2) {
+                      st_x[i] = xs[k];
+                      st_ox[i] = o0[k];
+                      st_dx[i] = o1[k] - o0[k];
+                    }
+                  }
+                }
+              }
+              if (lane == 0) {
+                if (!same) {
+                  s_task = tn;
+                  s_rsv = 1;
+                  s_hdr[0] = u2;
+                  s_hdr[1] = ou2;
+                  s_hdr[2] = du2;
+                  s_hdr[3] = ka2;
+                  s_hdr[4] = 0;
+                  s_hdr[5] = 0;
+                  s_hdr[6] = np2;
+                }
+                s_pre = staged ? 1 : 0;
+              }
+            }
+          }
           for (;;) {
             int un = 0;
             if (lane == 0) un = atomicAdd(&s_next, 1);

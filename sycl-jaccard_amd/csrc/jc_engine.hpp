@@ -302,13 +302,19 @@ struct QTable {
     split(y, b, r);
     for (int d = 0; d <= maxd; ++d) {
       const uint32_t key16 = r | ((uint32_t)d << rb);
-      volatile uint32_t* wb = word + 4 * b;
+      // the bucket in one 16-byte LDS read per attempt (volatile: re-read
+      // after a lost compare-and-swap; the LDS address space, or the generic
+      // pointer would make it a flat load)
+      typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+      typedef __attribute__((address_space(3))) const volatile u32x4 lds_bucket;
+      const lds_bucket* wb = (const lds_bucket*)(word + 4 * b);
       for (;;) {
+        const u32x4 w4 = *wb;
         int s = -1;
         uint32_t old = 0;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-          const uint32_t wv = wb[k];
+          const uint32_t wv = w4[k];
           if (s < 0 && (wv & 0xFFFFu) == 0xFFFFu) {
             s = 2 * k;
             old = wv;
@@ -1485,7 +1491,7 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
         if (owns(u, du, x, dx)) cls = len_class((dx + nch - 1) / nch);
       }
       int rank = 0;
-  #pragma unroll
+#pragma unroll
       for (int k = 0; k < kClasses; ++k) {
         const uint64_t bm = ballot(cls == k);
         if (cls == k) rank = popc64(bm & ((1ull << lane) - 1));
@@ -1589,10 +1595,10 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
             // stores are limited to the task
             constexpr int NI = (kTaskSpan + kWave - 1) / kWave;
             int32_t xs[NI];
-  #pragma unroll
+#pragma unroll
             for (int k = 0; k < NI; ++k) xs[k] = a.ind[ka2 + min(lane + k * kWave, span2 - 1)];
             int o0[NI], o1[NI];
-  #pragma unroll
+#pragma unroll
             for (int k = 0; k < NI; ++k) {
               o0[k] = a.off[xs[k]];
               o1[k] = a.off[xs[k] + 1];
@@ -1600,14 +1606,14 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
             if constexpr (W) {
               if (du2 <= kTaskSpan) {
                 float ws[NI];
-  #pragma unroll
+#pragma unroll
                 for (int k = 0; k < NI; ++k) ws[k] = a.vw[xs[k]];
-  #pragma unroll
+#pragma unroll
                 for (int k = 0; k < NI; ++k)
                   if (lane + k * kWave < span2) st_w[lane + k * kWave] = ws[k];
               }
             }
-  #pragma unroll
+#pragma unroll
             for (int k = 0; k < NI; ++k) {
               const int i = lane + k * kWave;
               if (i < span2) {

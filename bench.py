@@ -323,6 +323,64 @@ def run_served(args):
     return out
 
 
+def tuning_sweep(args):
+    """Every jc_tuning field over graph families (the data behind
+    jc_config_from_stats): one JSON line per (graph, configuration) with the
+    median call time of `reps` synchronized jc_jaccard_tuned calls and
+    whether the result equals the first configuration's bit for bit."""
+    import itertools
+    import numpy as np
+    import torch
+    from jaccard_amd import api, batch, graphs
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(0)
+    fields = ("small_max", "small_blocks_per_cu", "bitmap_blocks_per_cu", "hash_table")
+    full = list(itertools.product((128, 96, 64, 32), (8, 16, 32), (1, 2), (0, 1)))
+    small = list(itertools.product((128, 64), (16, 32), (1, 2), (0, 1)))
+    bb = [(128, 32, 1, 0), (128, 32, 2, 0)]
+
+    def run(name, g, weighted, grid, reps=3):
+        dg = api.DeviceCSR.from_host(g, dev)
+        W = api.vertex_weights(dg) if weighted else None
+        out = torch.empty(g.m, dtype=torch.float32, device=dev)
+        ws = torch.empty(api.workspace_size(g.n, g.m, weighted, True), dtype=torch.uint8, device=dev)
+        st = api.degree_stats(dg)
+        chosen = api.config_from_stats(st, g.n, weighted)[1]
+        print(f"[bench] sweep {name}: n={g.n} m={g.m} chosen {chosen}", file=sys.stderr, flush=True)
+        ref = None
+        for cand in grid:
+            t = dict(zip(fields, cand))
+            call = lambda: api.jaccard(dg, out, weights=W, workspace=ws, flags=api.FLAG_ASSUME_SYMMETRIC, tuning=t)
+            call()
+            torch.cuda.synchronize()
+            ts = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                call()
+                torch.cuda.synchronize()
+                ts.append((time.perf_counter() - t0) * 1e3)
+            bits = out.view(torch.int32).clone()
+            same = True if ref is None else bool(torch.equal(bits, ref))
+            ref = bits if ref is None else ref
+            print(json.dumps({"graph": name, "weighted": weighted, "n": g.n, "m": g.m, "tuning": t,
+                              "chosen": all(t[k] == chosen[k] for k in fields),
+                              "ms": sorted(ts)[len(ts) // 2], "same_bits": same}), flush=True)
+        del dg, W, out, ws
+        torch.cuda.empty_cache()
+
+    run("rmat16", graphs.rmat(16, 16, seed=1), False, full)
+    run("rmat18w", graphs.rmat(18, 16, seed=1, weighted=True), True, full)
+    run("rmat20", graphs.rmat(20, 16, seed=1), False, full)
+    run("chunglu20", graphs.chung_lu(1 << 20, seed=1), False, full)
+    run("rmat22w", graphs.rmat(22, 16, seed=1, weighted=True), True, small, reps=2)
+    for sc in (14, 16, 17, 19, 20):
+        run(f"rmat{sc}w", graphs.rmat(sc, 16, seed=1, weighted=True), True, bb)
+    run("chunglu18w", graphs.chung_lu(1 << 18, seed=1, weighted=True), True, bb)
+    bu, bw = batch.make_batches(count=args.batch_count, seed=1)
+    run("batch_unweighted", bu.union, False, small)
+    run("batch_weighted", bw.union, True, small)
+
+
 def run_ingest(args):
     """Graph ingest of the RMAT-22 ef16 draws (6.7e7 directed edges):
     jc_csr_from_edges on the device (inputs resident) vs the host C++/OpenMP
@@ -384,6 +442,8 @@ def main():
     ap.add_argument("--profile-batched", action="store_true",
                     help="only the batched form's captured step, --steps replays after autotuning "
                          "(for a kernel trace of it alone); prints one JSON line about it")
+    ap.add_argument("--tuning-sweep", action="store_true",
+                    help="only time every jc_tuning field over graph families (JSON line each)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="also run 3 device-to-device copies of BYTES (a known-size kernel that "
                          "calibrates FETCH_SIZE/WRITE_SIZE counters)")
@@ -400,6 +460,9 @@ def main():
 
     if args.profile_batched:
         print(json.dumps(profile_batched(args)))
+        return
+    if args.tuning_sweep:
+        tuning_sweep(args)
         return
 
     spec, weighted, desc = WORKLOADS[args.workload]

@@ -160,8 +160,8 @@ int jc_jaccard_tuned(const jc_csr *g, const float *vertex_weights, float *result
  * gives a bitmap table (w = 0 unweighted, 1 weighted layout, one workgroup
  * per CU): a row with first and last column lo, hi and degree d gets one when
  * W = (hi - lo) / 32 + 2 words of 4 bytes (weighted: W words of 8 bytes plus
- * 8 * (d + 1) bytes of weights) fit bitmap_table_bytes[w]; the largest
- * degree. */
+ * 8 * (d + 1) bytes of weights) fit bitmap_table_bytes[w] and, weighted, the
+ * 8 * W bytes of words are at most 32 KiB; the largest degree. */
 typedef struct jc_graph_stats {
   int64_t rows[32];
   int64_t entries[32];

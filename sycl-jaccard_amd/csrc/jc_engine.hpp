@@ -1179,12 +1179,21 @@ constexpr size_t kBlockLds = JC_BLOCK_LDS_KIB * 1024 - 1024;  // minus the kerne
 
 // Table kind of a long row, from its first and last column: the bitmap when
 // its words and the guard word (weighted: with their ranks) and, weighted,
-// the zero slot and the keys' fp64 weights fit the table region of the LDS. Used by the scheduler and
+// the zero slot and the keys' fp64 weights fit the table region of the LDS,
+// and, weighted, the words and ranks take at most kBitmapWordsMaxW bytes:
+// a weighted bitmap over a wider id span (a sparse row) is cleared, ranked
+// and probed more slowly than a quotient table of its keys (measured: RMAT
+// and Chung-Lu graphs of 2^18 vertices 26-30% faster on quotient tables,
+// 2^14-2^16 vertices 2-2.5x faster on bitmaps, 2^17 even; unweighted rows
+// are faster on bitmaps at every span that fits). Used by the scheduler and
 // by the task kernels alike.
+constexpr size_t kBitmapWordsMaxW = 32 * 1024;
 template <bool W>
 __host__ __device__ __forceinline__ bool bitmap_fits(int32_t lo, int32_t hi, int du, size_t budget) {
   const uint32_t words = ((uint32_t)(hi - lo) >> 5) + 1u;  // plus one zero guard word
-  return JC_BITMAP && (size_t)(words + 1u) * (W ? 8u : 4u) + (W ? (size_t)8 * (du + 1) : 0) <= budget;
+  const size_t word_bytes = (size_t)(words + 1u) * (W ? 8u : 4u);
+  return JC_BITMAP && word_bytes + (W ? (size_t)8 * (du + 1) : 0) <= budget &&
+         (!W || word_bytes <= kBitmapWordsMaxW);
 }
 
 // Exclusive prefix sum over the workgroup (all threads active).

@@ -15,7 +15,8 @@ def host_bitmap_rows(g, table_bytes):
     hi = g.indices[off[longr + 1] - 1].astype(np.int64)
     words = (hi - lo) // 32 + 2
     d = deg[longr]
-    return int((4 * words <= table_bytes[0]).sum()), int((8 * words + 8 * (d + 1) <= table_bytes[1]).sum())
+    return (int((4 * words <= table_bytes[0]).sum()),
+            int(((8 * words + 8 * (d + 1) <= table_bytes[1]) & (8 * words <= 32 * 1024)).sum()))
 
 
 def host_stats(g):

@@ -43,7 +43,8 @@ WORKLOADS = {
 # rocprofv3 summary of the dominant kernel's HBM traffic per workload: bytes
 # from the TCC read/write request counts by request size (see the file for
 # the commands and the copy-kernel check).
-TRAFFIC_PROFILES = {"rmat22_weighted": "profiles/owner_block_rmat22_weighted_pmc.md"}
+TRAFFIC_PROFILES = {"rmat22_weighted": "profiles/owner_block_rmat22_weighted_pmc.md",
+                    "chunglu20": "profiles/owner_block_chunglu20_pmc.md"}
 
 
 def profile_value(workload, key, cast=float):

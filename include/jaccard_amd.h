@@ -182,15 +182,18 @@ int jc_degree_stats(const jc_csr *g, jc_graph_stats *stats, jc_stream_t stream);
  *     arrays (weight_i, weight_s) and runs the jw pass;
  *   - small_max 128: every row up to the largest the one-wave-per-row
  *     kernel's table holds goes to that kernel (smaller bins were slower in
- *     every family measured, up to 2.4x);
+ *     every family measured, up to 1.4x);
  *   - small_blocks_per_cu 32 (best or within 1% in every family);
  *   - bitmap_blocks_per_cu 1 (two workgroups per CU shrink the bitmap table
- *     and were up to 6% slower);
+ *     and were up to 2.4x slower);
  *   - hash_table 0 (quotient tables) below 2^24 vertices, 1 (32-bit tables)
  *     from there on.
- * These come from a sweep of every jc_tuning field over seven graph families
- * (RMAT 2^16 to 2^22, weighted and not, Chung-Lu 2^20, the two halves of a
- * 4096-graph batch); no field changes results. */
+ * These come from a sweep of every jc_tuning field over thirteen graph
+ * families (RMAT 2^14 to 2^22, weighted and not, Chung-Lu 2^18 and 2^20, the
+ * two halves of a 4096-graph batch), where the chosen configuration was the
+ * fastest or within 1% of it on eleven; no field changes results. The
+ * table kind of each row (bitmap or hash) is decided per row from its
+ * degree and column span by the scheduler, as jc_degree_stats reports. */
 int jc_config_from_stats(const jc_graph_stats *stats, int32_t n, int weighted, int *edge_centric,
                          jc_tuning *tuning);
 

@@ -95,3 +95,20 @@ def test_auto_choice_and_compare(tmp_path, cuda):
     cmp = subprocess.run([os.path.join(BIN, "compare_coords"), str(tmp_path / "ec.csr"), str(tmp_path / "auto.csr"), "0"],
                          capture_output=True, text=True)
     assert cmp.returncode == 0
+
+
+def test_auto_weighted_matches_forced_vc(tmp_path, cuda):
+    """With no FORCE variable the driver asks jc_select_config: on a weighted
+    graph it runs weighted edge-centric Jaccard with the chosen tuning, whose
+    bits equal the forced vertex-centric run's (the same sums and division)."""
+    g = graphs.rmat(11, 8, seed=12, weighted=True)
+    src = str(tmp_path / "g.csr")
+    fileio.save(src, g)
+    r_auto = run(src, tmp_path / "auto.csr")
+    r_vc = run(src, tmp_path / "vc.csr", env={"JACCARD_FORCE_VERTEX_CENTRIC": "1"})
+    assert r_auto.returncode == 0 and r_vc.returncode == 0, (r_auto.stderr, r_vc.stderr)
+    assert "AUTO Edge-Centric Implementation (weighted) small_max 128" in r_auto.stderr
+    assert "non zero pairs" in r_auto.stdout
+    a = fileio.load(str(tmp_path / "auto.csr")).graph.weights
+    b = fileio.load(str(tmp_path / "vc.csr")).graph.weights
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))

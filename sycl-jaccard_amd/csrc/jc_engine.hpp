@@ -1474,7 +1474,12 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
     // bitmap tasks: the row's id span fits the table region of the LDS
     // (bitmap_fits, decided by the scheduler)
     const uint32_t words = ((uint32_t)(hi0 - lo0) >> 5) + 1u;
+    // rows longer than one chunk: nch chunks of equal size (a full chunk
+    // and a short last one would fill the first chunk's buckets to the
+    // limit, so more of its keys overflow into neighbouring buckets and
+    // more probes walk the overflow)
     const int nch = KIND == 1 ? 1 : (du + CH - 1) / CH;
+    const int chk = (du + nch - 1) / nch;
     bool built = true;  // hash-table tasks: false once handed to the 32-bit redo pass
     for (int pc = 0; pc < npieces && built; ++pc) {
       if (pc > 0) {
@@ -1666,8 +1671,8 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
       if constexpr (KIND != 1) {
         constexpr int TK = kQ ? kTkQuot : kTkHash;
         for (int c = 0; c < nch; ++c) {
-          const int cs = ou + c * CH;
-          const int ce = min(cs + CH, ou + du);
+          const int cs = ou + c * chk;
+          const int ce = min(cs + chk, ou + du);
           const int clen = ce - cs;
           // a single-chunk table stays in the LDS for the task's later pieces
           const bool fresh = nch > 1 || pc == 0;

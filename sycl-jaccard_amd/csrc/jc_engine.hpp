@@ -230,6 +230,18 @@ __device__ __forceinline__ int table_log_buckets(int len) {
   return max(ceil_log2((len + kLambda<W> - 1) / kLambda<W>), 2);
 }
 
+// The same for the workgroup kernel, whose table region always holds max_lb
+// buckets: tables of chunks shorter than a full one get up to one bucket per
+// key (fewer buckets overflow into their neighbours, which costs the whole
+// wave an overflow walk on every pass where one lane meets one).
+#ifndef JC_TABLE_LOWFILL
+#define JC_TABLE_LOWFILL 1
+#endif
+template <bool W>
+__device__ __forceinline__ int block_table_log_buckets(int len, int max_lb) {
+  return JC_TABLE_LOWFILL ? min(max(ceil_log2(len), 2), max_lb) : table_log_buckets<W>(len);
+}
+
 // Inserts keys ind[cs .. cs+len) (weighted: with vw[key]) into the table,
 // `stride` threads cooperating; four keys per thread are loaded before any
 // insert so the loads overlap.
@@ -359,9 +371,17 @@ struct QTable {
   }
 };
 
-// log2 of the bucket count of a quotient table for a chunk of len keys.
-__device__ __forceinline__ int qtable_log_buckets(int len) {
-  return max(ceil_log2((len + kQLambda - 1) / kQLambda), kQMinLb);
+// log2 of the bucket count of a quotient table for a chunk of len keys, at
+// most max_lb (the table region's buckets). A table is sized for a mean of
+// at most JC_QFILL2 / 2 keys per bucket where the region allows it: the
+// fuller the buckets, the more of them overflow into a neighbour, and a pass
+// in which any lane's bucket has overflowed walks the overflow for the whole
+// wave. (Chunks are cut at kQLambda keys per bucket of the whole region.)
+#ifndef JC_QFILL2
+#define JC_QFILL2 2
+#endif
+__device__ __forceinline__ int qtable_log_buckets(int len, int max_lb) {
+  return min(max(ceil_log2((2 * len + JC_QFILL2 - 1) / JC_QFILL2), kQMinLb), max_lb);
 }
 
 // ---------------------------------------------------------------------------
@@ -1678,7 +1698,7 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
           const bool fresh = nch > 1 || pc == 0;
           Chunk<W> cx;
           if constexpr (kQ) {
-            const int lb = qtable_log_buckets(clen);
+            const int lb = qtable_log_buckets(clen, __builtin_ctz(Cfg::QNB));
             const int maxd = (1 << (lb - 8)) - 2;  // 2^(16 - rb) - 2
             cx.qtab = QTable<W>{qword, qval, lb, (uint32_t)((1 << lb) - 1), 24 - lb,
                                 a.quot_max_disp > 0 ? min(maxd, a.quot_max_disp - 1) : maxd};
@@ -1700,7 +1720,7 @@ __global__ __launch_bounds__(BLOCK) void k_owner_block(EngineArgs a) {
               }
             }
           } else {
-            const int lb = table_log_buckets<W>(clen);
+            const int lb = block_table_log_buckets<W>(clen, __builtin_ctz(Cfg::NB));
             cx.tab = Table<W>{tkey, tval, lb, (uint32_t)((1 << lb) - 1), Cfg::NB * 4};
             if (fresh) {
               for (int i = tid; i < (4 << lb); i += BLOCK) cx.tab.clear(i);

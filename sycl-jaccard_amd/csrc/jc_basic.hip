@@ -89,10 +89,11 @@ __global__ void k_rowsum_deg(int32_t n, const int32_t* __restrict__ off, float* 
 
 // Weighted row sum over column weights: sum_{k in row u} v[ind[k]] (ind may be
 // nullptr: identity index, i.e. the row's own edge weights, as main.cpp:216-257
-// arranges with its presumInd buffer). Rows up to kRowWave entries: groups
-// of 8 lanes or whole waves (k_rowsum_w); longer rows: one workgroup per row
-// (second kernel), so a hub row is not a serial tail. Loads are unrolled 4x
-// per lane.
+// arranges with its presumInd buffer). By row length: rows of at most
+// kRowShort entries by groups of 4 lanes (k_rowsum_w), rows up to kRowWave
+// entries one wave per row (in k_rowsum_w, or from a list in k_rowsum_w_med
+// when the sum gathers), longer rows one workgroup per row from a list
+// (k_rowsum_w_long). Loads are unrolled 4x per lane.
 #ifndef JC_ROW_WAVE
 #define JC_ROW_WAVE 2048
 #endif
@@ -119,14 +120,23 @@ __device__ __forceinline__ void gather_step(const int32_t* __restrict__ ind, con
     if (p0 + t * stride + lane >= end) x[t] = 0.0f;
 }
 
-// Rows up to kRowWave entries, 8 consecutive rows per wave and iteration:
-// a row of at most kRowShort entries (most rows of power-law graphs) is
-// summed by its own group of 8 lanes; the longer ones of the 8 are then
-// summed one after another by the whole wave. (One wave per row leaves most
-// lanes idle and pays a row's load latency per row.) The double-double sums
-// are exact, so the lane split does not change the result.
+// Rows of at most kRowShort entries (most rows of power-law graphs), 16
+// consecutive rows per wave and iteration, each summed by its own group of 4
+// lanes (groups of 8 measured the same or slightly slower). (One wave per row
+// leaves most lanes idle and pays a row's load latency per row.) Rows of up to kRowWave entries: with a gather (IDX, the
+// sum of v over the row's columns) they are listed by k_collect_rows and
+// summed one wave per row by k_rowsum_w_med; summed here one after another by
+// the whole wave, the dependent gathers made the waves that met many of them
+// (power-law graphs put them at structured ids) the kernel's tail: RMAT-22
+// 1.52 -> 1.14 ms. Without one (the row's own edge weights) the whole wave
+// sums them here, which measured faster than the list (0.63 against 0.72 ms).
+// The double-double sums are exact, so the lane split does not change the
+// result.
 constexpr int kRowShort = 64;
-constexpr int kRowGroup = 8;
+#ifndef JC_ROW_GROUP
+#define JC_ROW_GROUP 4
+#endif
+constexpr int kRowGroup = JC_ROW_GROUP;
 
 template <typename OutT, bool IDX>
 __global__ void k_rowsum_w(int32_t n, const int32_t* __restrict__ off, const int32_t* __restrict__ ind,
@@ -157,35 +167,91 @@ __global__ void k_rowsum_w(int32_t n, const int32_t* __restrict__ off, const int
         acc = add_dd(acc, DD2{__shfl_xor(acc.hi, sh), __shfl_xor(acc.lo, sh)});
       if (short_row && gl == 0) out[u] = (OutT)acc.hi;
     }
-    uint64_t med = __ballot(gl == 0 && u < n && d > kRowShort && d <= kRowWave);
-    while (med) {
-      const int src = __ffsll((unsigned long long)med) - 1;
-      med &= med - 1;
-      const int32_t uu = __shfl(u, src);
-      const int oo = __shfl(o, src), dd = __shfl(d, src);
-      DD2 acc{0.0, 0.0};
-      for (int p0 = oo; p0 < oo + dd; p0 += kWave * kRowU) {
-        float x[kRowU];
-        gather_step<IDX, kRowU>(ind, v, p0, kWave, lane, oo + dd, x);
+    if constexpr (!IDX) {
+      uint64_t med = __ballot(gl == 0 && u < n && d > kRowShort && d <= kRowWave);
+      while (med) {
+        const int src = __ffsll((unsigned long long)med) - 1;
+        med &= med - 1;
+        const int32_t uu = __shfl(u, src);
+        const int oo = __shfl(o, src), dd = __shfl(d, src);
+        DD2 acc{0.0, 0.0};
+        for (int p0 = oo; p0 < oo + dd; p0 += kWave * kRowU) {
+          float x[kRowU];
+          gather_step<IDX, kRowU>(ind, v, p0, kWave, lane, oo + dd, x);
 #pragma unroll
-        for (int t = 0; t < kRowU; ++t) acc = add_d(acc, (double)x[t]);
+          for (int t = 0; t < kRowU; ++t) acc = add_d(acc, (double)x[t]);
+        }
+#pragma unroll
+        for (int sh = 1; sh < kWave; sh <<= 1)
+          acc = add_dd(acc, DD2{__shfl_xor(acc.hi, sh), __shfl_xor(acc.lo, sh)});
+        if (lane == 0) out[uu] = (OutT)acc.hi;
       }
-#pragma unroll
-      for (int sh = 1; sh < kWave; sh <<= 1) acc = add_dd(acc, DD2{__shfl_xor(acc.hi, sh), __shfl_xor(acc.lo, sh)});
-      if (lane == 0) out[uu] = (OutT)acc.hi;
     }
   }
 }
 
-// Rows longer than kRowWave: collected into a list, then taken one at a time
-// by persistent 1024-thread workgroups from an atomic counter. (A static
-// row-to-workgroup map is not used: in power-law graphs the long rows sit at
-// structured ids -- RMAT hubs have many trailing zero bits -- and any fixed
-// stride piles them onto a few workgroups.)
-__global__ void k_collect_long(int32_t n, const int32_t* __restrict__ off, int* __restrict__ list,
-                               int* __restrict__ counters) {
-  for (int32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x)
-    if (off[u + 1] - off[u] > kRowWave) list[atomicAdd(&counters[0], 1)] = u;
+// Rows of kRowShort + 1 .. kRowWave entries, one wave per row; wave w of the
+// grid takes the list's entries w, w + waves, ..., and the grid has several
+// times the waves the device holds at once, so the dispatcher balances the
+// rows' unequal lengths (an atomic cursor shared by every wave serialises on
+// its one address: measured slower). The list is stored from the end of
+// `list` backwards (k_collect_rows).
+#ifndef JC_MED_U
+#define JC_MED_U 4
+#endif
+constexpr int kMedU = JC_MED_U;
+
+template <typename OutT, bool IDX>
+__global__ void k_rowsum_w_med(int32_t n, const int32_t* __restrict__ off, const int32_t* __restrict__ ind,
+                               const float* __restrict__ v, OutT* __restrict__ out, const int* __restrict__ list,
+                               const int* __restrict__ counters) {
+  const int lane = lane_id();
+  const int count = counters[2];
+  const int nw = gridDim.x * (blockDim.x / kWave);
+  for (int r = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; r < count; r += nw) {
+    const int32_t u = list[n - 1 - r];
+    const int o = off[u], d = off[u + 1] - o;
+    DD2 acc{0.0, 0.0};
+    for (int p0 = o; p0 < o + d; p0 += kWave * kMedU) {
+      float x[kMedU];
+      gather_step<IDX, kMedU>(ind, v, p0, kWave, lane, o + d, x);
+#pragma unroll
+      for (int t = 0; t < kMedU; ++t) acc = add_d(acc, (double)x[t]);
+    }
+#pragma unroll
+    for (int sh = 1; sh < kWave; sh <<= 1) acc = add_dd(acc, DD2{__shfl_xor(acc.hi, sh), __shfl_xor(acc.lo, sh)});
+    if (lane == 0) out[u] = (OutT)acc.hi;
+  }
+}
+
+// Rows longer than kRowWave fill list[0 ..) (counters[0]), taken from an
+// atomic cursor (counters[1]) by a persistent grid of 1024-thread
+// workgroups, one row each. (A static row-to-workgroup map is not used: in
+// power-law graphs the long rows sit at structured ids -- RMAT hubs have many
+// trailing zero bits -- and any fixed stride piles them onto a few
+// workgroups.) With list_med, rows of kRowShort + 1 .. kRowWave entries fill
+// list[n-1] downwards (counters[2]) for k_rowsum_w_med. One wave-aggregated
+// atomic per list and wave; the lists are disjoint, so n entries hold both.
+__global__ void k_collect_rows(int32_t n, const int32_t* __restrict__ off, int* __restrict__ list,
+                               int* __restrict__ counters, int list_med) {
+  const int lane = lane_id();
+  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  for (int32_t base = (blockIdx.x * blockDim.x + threadIdx.x) & ~(kWave - 1); base < n;
+       base += gridDim.x * blockDim.x) {
+    const int32_t u = base + lane;
+    const int d = u < n ? off[u + 1] - off[u] : 0;
+    const uint64_t ml = __ballot(d > kRowWave);
+    const uint64_t mm = __ballot(list_med && d > kRowShort && d <= kRowWave);
+    int bl = 0, bm = 0;
+    if (lane == 0) {
+      if (ml) bl = atomicAdd(&counters[0], __popcll(ml));
+      if (mm) bm = atomicAdd(&counters[2], __popcll(mm));
+    }
+    bl = __shfl(bl, 0);
+    bm = __shfl(bm, 0);
+    if (d > kRowWave) list[bl + __popcll(ml & below)] = u;
+    else if (list_med && d > kRowShort) list[n - 1 - (bm + __popcll(mm & below))] = u;
+  }
 }
 
 template <typename OutT, bool IDX, int BLOCK>
@@ -227,7 +293,7 @@ __global__ __launch_bounds__(BLOCK) void k_rowsum_w_long(const int32_t* __restri
   }
 }
 
-size_t rowsum_scratch_bytes(int32_t n) { return ((size_t)std::max(n, 1) + 2) * sizeof(int); }
+size_t rowsum_scratch_bytes(int32_t n) { return ((size_t)std::max(n, 1) + 4) * sizeof(int); }
 
 // scratch: rowsum_scratch_bytes(n) bytes, or nullptr to use a stream-ordered
 // allocation for the call.
@@ -241,13 +307,17 @@ static hipError_t rowsum_both(int32_t n, const int32_t* off, const int32_t* ind,
     scratch = own;
   }
   int* counters = static_cast<int*>(scratch);
-  int* list = counters + 2;
+  int* list = counters + 4;
   const int cus = device_cu_count();
-  hipLaunchKernelGGL(k_fill<uint32_t>, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t*>(counters), (int64_t)2, 0u);
-  hipLaunchKernelGGL(k_collect_long, dim3(grid_for(n, 256, 8)), dim3(256), 0, s, n, off, list, counters);
+  hipLaunchKernelGGL(k_fill<uint32_t>, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t*>(counters), (int64_t)4, 0u);
+  hipLaunchKernelGGL(k_collect_rows, dim3(grid_for(n, 256, 8)), dim3(256), 0, s, n, off, list, counters,
+                     ind ? 1 : 0);
   const int gs = grid_rows((n + kWave / kRowGroup - 1) / (kWave / kRowGroup), 4, 64);
+  const int gm = grid_rows(n, 4, 64);  // at most n listed rows, one per wave
   if (ind) {
     hipLaunchKernelGGL((k_rowsum_w<OutT, true>), dim3(gs), dim3(256), 0, s, n, off, ind, v, out);
+    hipLaunchKernelGGL((k_rowsum_w_med<OutT, true>), dim3(gm), dim3(256), 0, s, n, off, ind, v, out, list,
+                       counters);
     hipLaunchKernelGGL((k_rowsum_w_long<OutT, true, 1024>), dim3(cus * 2), dim3(1024), 0, s, off, ind, v, out,
                        list, counters);
   } else {

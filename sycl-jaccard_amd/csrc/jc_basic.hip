@@ -567,7 +567,21 @@ __global__ __launch_bounds__(256) void k_exact_scan(int32_t n, const float* __re
     mx = max(mx, __shfl_xor(mx, o));
     bad |= __shfl_xor(bad, o);
   }
+  // one set of atomics per workgroup (per wave, the few counters serialise)
+  __shared__ int s_r[3][256 / kWave];
+  const int w = threadIdx.x / kWave;
   if (lane_id() == 0) {
+    s_r[0][w] = lsb;
+    s_r[1][w] = mx;
+    s_r[2][w] = bad;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < (int)(blockDim.x / kWave); ++k) {
+      lsb = min(lsb, s_r[0][k]);
+      mx = max(mx, s_r[1][k]);
+      bad |= s_r[2][k];
+    }
     if (lsb != INT32_MAX) atomicMin(&info->min_lsb, lsb);
     if (mx != INT32_MIN) atomicMax(&info->max_exp, mx);
     if (bad) atomicOr(&info->bad, 1);

@@ -143,6 +143,13 @@ def test_asymmetric_fixup(ec, cuda):
     v = rng.random(g.n).astype(np.float32) + 0.5
     ref = oracle.weighted_f64(g, v.astype(np.float64))
     assert rel_err(run(g, cuda, v, ec=ec), ref).max() <= WEIGHTED_RTOL
+    # a result view that is not 16-byte aligned (the fix-up pass scans it)
+    import torch
+    buf = torch.empty(g.m + 1, dtype=torch.float32, device=cuda)
+    out = buf[1:]
+    assert out.data_ptr() % 16 != 0
+    api.jaccard(dg, out, edge_centric=ec)
+    assert_bits_equal(out.cpu().numpy(), oracle.ec_unweighted(g), "asymmetric, unaligned result")
 
 
 def test_self_loops(cuda):
@@ -207,13 +214,15 @@ def test_pairs(cuda):
 
 
 def test_row_sums_at_length_boundaries(cuda):
-    """Row sums route rows by length (8 lanes up to 64 entries, a whole wave
-    up to 2048, a workgroup beyond); rows on both sides of each limit must
-    give the exact sums: vertex weights (edge weights of the row) and the
-    weighted RowSum (vertex weights gathered over the row's columns)."""
+    """Row sums route rows by length (4 lanes up to 64 entries, 16 per step;
+    a whole wave up to 2048, in the same kernel for edge weights and in its
+    own for gathered sums; a workgroup beyond); rows on both sides of each
+    limit must give the exact sums: vertex weights (edge weights of the row)
+    and the weighted RowSum (vertex weights gathered over the row's
+    columns)."""
     import torch
     rng = np.random.default_rng(9)
-    degs = [0, 1, 7, 8, 9, 63, 64, 65, 66, 2047, 2048, 2049, 5000, 3, 0, 129]
+    degs = [0, 1, 7, 8, 9, 15, 16, 17, 32, 33, 63, 64, 65, 66, 2047, 2048, 2049, 5000, 3, 0, 129, 300, 1000]
     n = 6000
     rows = []
     for r, d in enumerate(degs):

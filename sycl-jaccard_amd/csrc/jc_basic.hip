@@ -91,9 +91,9 @@ __global__ void k_rowsum_deg(int32_t n, const int32_t* __restrict__ off, float* 
 // nullptr: identity index, i.e. the row's own edge weights, as main.cpp:216-257
 // arranges with its presumInd buffer). By row length: rows of at most
 // kRowShort entries by groups of 4 lanes (k_rowsum_w), rows up to kRowWave
-// entries one wave per row (in k_rowsum_w, or from a list in k_rowsum_w_med
-// when the sum gathers), longer rows one workgroup per row from a list
-// (k_rowsum_w_long). Loads are unrolled 4x per lane.
+// entries one wave per row from a list (k_rowsum_w_med), longer rows one
+// workgroup per row from a list (k_rowsum_w_long). Loads are unrolled 4x per
+// lane.
 #ifndef JC_ROW_WAVE
 #define JC_ROW_WAVE 2048
 #endif
@@ -123,15 +123,13 @@ __device__ __forceinline__ void gather_step(const int32_t* __restrict__ ind, con
 // Rows of at most kRowShort entries (most rows of power-law graphs), 16
 // consecutive rows per wave and iteration, each summed by its own group of 4
 // lanes (groups of 8 measured the same or slightly slower). (One wave per row
-// leaves most lanes idle and pays a row's load latency per row.) Rows of up to kRowWave entries: with a gather (IDX, the
-// sum of v over the row's columns) they are listed by k_collect_rows and
-// summed one wave per row by k_rowsum_w_med; summed here one after another by
-// the whole wave, the dependent gathers made the waves that met many of them
-// (power-law graphs put them at structured ids) the kernel's tail: RMAT-22
-// 1.52 -> 1.14 ms. Without one (the row's own edge weights) the whole wave
-// sums them here, which measured faster than the list (0.63 against 0.72 ms).
-// The double-double sums are exact, so the lane split does not change the
-// result.
+// leaves most lanes idle and pays a row's load latency per row.) Rows of up
+// to kRowWave entries are listed by k_collect_rows and summed one wave per
+// row by k_rowsum_w_med: summed here one after another by the whole wave,
+// they made the waves that met many of them (power-law graphs put them at
+// structured ids) the kernel's tail (RMAT-22: gathered sums 1.52 -> 0.83 ms,
+// vertex weights 0.63 -> 0.41 ms with the lists). The double-double sums are
+// exact, so the lane split does not change the result.
 constexpr int kRowShort = 64;
 #ifndef JC_ROW_GROUP
 #define JC_ROW_GROUP 4
@@ -166,26 +164,6 @@ __global__ void k_rowsum_w(int32_t n, const int32_t* __restrict__ off, const int
       for (int sh = 1; sh < kRowGroup; sh <<= 1)
         acc = add_dd(acc, DD2{__shfl_xor(acc.hi, sh), __shfl_xor(acc.lo, sh)});
       if (short_row && gl == 0) out[u] = (OutT)acc.hi;
-    }
-    if constexpr (!IDX) {
-      uint64_t med = __ballot(gl == 0 && u < n && d > kRowShort && d <= kRowWave);
-      while (med) {
-        const int src = __ffsll((unsigned long long)med) - 1;
-        med &= med - 1;
-        const int32_t uu = __shfl(u, src);
-        const int oo = __shfl(o, src), dd = __shfl(d, src);
-        DD2 acc{0.0, 0.0};
-        for (int p0 = oo; p0 < oo + dd; p0 += kWave * kRowU) {
-          float x[kRowU];
-          gather_step<IDX, kRowU>(ind, v, p0, kWave, lane, oo + dd, x);
-#pragma unroll
-          for (int t = 0; t < kRowU; ++t) acc = add_d(acc, (double)x[t]);
-        }
-#pragma unroll
-        for (int sh = 1; sh < kWave; sh <<= 1)
-          acc = add_dd(acc, DD2{__shfl_xor(acc.hi, sh), __shfl_xor(acc.lo, sh)});
-        if (lane == 0) out[uu] = (OutT)acc.hi;
-      }
     }
   }
 }
@@ -229,7 +207,7 @@ __global__ void k_rowsum_w_med(int32_t n, const int32_t* __restrict__ off, const
 // workgroups, one row each. (A static row-to-workgroup map is not used: in
 // power-law graphs the long rows sit at structured ids -- RMAT hubs have many
 // trailing zero bits -- and any fixed stride piles them onto a few
-// workgroups.) With list_med, rows of kRowShort + 1 .. kRowWave entries fill
+// workgroups.) Rows of kRowShort + 1 .. kRowWave entries fill
 // list[n-1] downwards (counters[2]) for k_rowsum_w_med. The lists are
 // disjoint, so n entries hold both. Each workgroup takes a contiguous range
 // of rows: it counts them, scans the counts over its threads and reserves
@@ -237,8 +215,7 @@ __global__ void k_rowsum_w_med(int32_t n, const int32_t* __restrict__ off, const
 // one counter serialised: 0.22 ms on RMAT-22).
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_collect_rows(int32_t n, const int32_t* __restrict__ off,
-                                                        int* __restrict__ list, int* __restrict__ counters,
-                                                        int list_med) {
+                                                        int* __restrict__ list, int* __restrict__ counters) {
   constexpr int NW = BLOCK / kWave;
   __shared__ int s_w[2][NW];
   __shared__ int s_base[2];
@@ -251,7 +228,7 @@ __global__ __launch_bounds__(BLOCK) void k_collect_rows(int32_t n, const int32_t
   for (int32_t u = r0 + threadIdx.x; u < r1; u += BLOCK) {
     const int d = off[u + 1] - off[u];
     cl += d > kRowWave ? 1 : 0;
-    cm += list_med && d > kRowShort && d <= kRowWave ? 1 : 0;
+    cm += d > kRowShort && d <= kRowWave ? 1 : 0;
   }
   // inclusive scans over the wave, then over the waves
   int il = cl, im = cm;
@@ -286,7 +263,7 @@ __global__ __launch_bounds__(BLOCK) void k_collect_rows(int32_t n, const int32_t
   for (int32_t u = r0 + threadIdx.x; u < r1; u += BLOCK) {
     const int d = off[u + 1] - off[u];
     if (d > kRowWave) list[pl++] = u;
-    else if (list_med && d > kRowShort) list[n - 1 - pm++] = u;
+    else if (d > kRowShort) list[n - 1 - pm++] = u;
   }
 }
 
@@ -346,8 +323,7 @@ static hipError_t rowsum_both(int32_t n, const int32_t* off, const int32_t* ind,
   int* list = counters + 4;
   const int cus = device_cu_count();
   hipLaunchKernelGGL(k_fill<uint32_t>, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t*>(counters), (int64_t)4, 0u);
-  hipLaunchKernelGGL(k_collect_rows<256>, dim3(grid_for(n, 256 * 4, 4)), dim3(256), 0, s, n, off, list, counters,
-                     ind ? 1 : 0);
+  hipLaunchKernelGGL(k_collect_rows<256>, dim3(grid_for(n, 256 * 4, 4)), dim3(256), 0, s, n, off, list, counters);
   const int gs = grid_rows((n + kWave / kRowGroup - 1) / (kWave / kRowGroup), 4, 64);
   const int gm = grid_rows(n, 4, 64);  // at most n listed rows, one per wave
   if (ind) {
@@ -358,6 +334,8 @@ static hipError_t rowsum_both(int32_t n, const int32_t* off, const int32_t* ind,
                        list, counters);
   } else {
     hipLaunchKernelGGL((k_rowsum_w<OutT, false>), dim3(gs), dim3(256), 0, s, n, off, ind, v, out);
+    hipLaunchKernelGGL((k_rowsum_w_med<OutT, false>), dim3(gm), dim3(256), 0, s, n, off, ind, v, out, list,
+                       counters);
     hipLaunchKernelGGL((k_rowsum_w_long<OutT, false, 1024>), dim3(cus * 2), dim3(1024), 0, s, off, ind, v, out,
                        list, counters);
   }

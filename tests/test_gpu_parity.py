@@ -215,8 +215,7 @@ def test_pairs(cuda):
 
 def test_row_sums_at_length_boundaries(cuda):
     """Row sums route rows by length (4 lanes up to 64 entries, 16 per step;
-    a whole wave up to 2048, in the same kernel for edge weights and in its
-    own for gathered sums; a workgroup beyond); rows on both sides of each
+    a whole wave up to 2048; a workgroup beyond); rows on both sides of each
     limit must give the exact sums: vertex weights (edge weights of the row)
     and the weighted RowSum (vertex weights gathered over the row's
     columns)."""

@@ -121,15 +121,6 @@ __device__ __forceinline__ bool exact_f64(const EngineArgs& a) {
 #ifndef JC_PINGPONG
 #define JC_PINGPONG 1  // row streaming loop unrolled twice (two register sets, no copies)
 #endif
-// c ? a : b per lane as one v_cndmask_b32 that the compiler cannot turn into
-// a branch (it would otherwise sink the computation of a into one).
-__device__ __forceinline__ uint32_t lane_select(bool c, uint32_t a, uint32_t b) {
-  const uint64_t m = __builtin_amdgcn_ballot_w64(c);
-  uint32_t r;
-  asm("v_cndmask_b32 %0, %1, %2, %3" : "=v"(r) : "v"(b), "v"(a), "s"(m));
-  return r;
-}
-
 constexpr int kBucket = 8;
 constexpr int kBucketKeys = 7;
 constexpr int32_t kSpill = -2;
@@ -205,7 +196,9 @@ struct Table {
     if constexpr (W) {
       const bool m1 = e1 | e3 | e5, m2 = e2 | e3 | e6, m4 = e4 | e5 | e6;
       const uint32_t slot = (m1 ? 1u : 0u) + (m2 ? 2u : 0u) + (m4 ? 4u : 0u);
-      h.v = val[lane_select(h.hit, b * kBucket + slot, (uint32_t)kBucketKeys)];
+      uint32_t idx = b * kBucket + slot;
+      asm volatile("" : "+v"(idx));  // computed on every lane (see QTable::match)
+      h.v = val[h.hit ? idx : (uint32_t)kBucketKeys];
     }
     return h;
   }
@@ -363,9 +356,14 @@ struct QTable {
       const bool m1 = e1 | e3 | e5, m2 = e2 | e3 | e6, m4 = e4 | e5 | e6;
       const uint32_t slot = (m1 ? 1u : 0u) + (m2 ? 2u : 0u) + (m4 ? 4u : 0u);
       const uint32_t base = __builtin_amdgcn_ubfe(w.w, 16, 15);
-      // lane_select, not ?: (which the compiler turns into a branch around
-      // the load, measured 6% slower on RMAT-22 weighted)
-      h.v = val[lane_select(h.hit, 1u + base + slot, 0u)];
+      uint32_t idx = 1u + base + slot;
+      // The index is computed on every lane before the select: without the
+      // empty asm the compiler sinks it and the load into a branch (measured
+      // 6% slower on RMAT-22 weighted). The select is then one v_cndmask on
+      // the compares' lane mask; a ballot-based select went through a VGPR
+      // and a compare, two more instructions per probe (60.1 -> 58.7 ms).
+      asm volatile("" : "+v"(idx));
+      h.v = val[h.hit ? idx : 0u];
     }
     return h;
   }

@@ -153,6 +153,50 @@ def oracle_check(g, out, W, weighted, n_sample=200000, seed=3):
             "reference": "oracle restatement of the reference edge-centric loop (jaccard.cpp:711-742)"}
 
 
+def run_kernel_level(dg, W, out):
+    """The reference's own kernel sequences through the kernel-level entry
+    points, on the bench graph: EC as Jaccard_ec_scan + Jaccard_ec_unweighted
+    (jaccard.cpp:654-782), VC as RowSum + Fill + Is + Jw (jaccard.cpp:858-896,
+    fp32 buffers as the reference keeps them). Median of 3 synchronized runs;
+    the EC result is compared bit for bit with the fused unweighted call."""
+    import torch
+    from jaccard_amd import api
+    m, n, dev = dg.m, dg.n, dg.offsets.device
+    dest = torch.empty(m, dtype=torch.int32, device=dev)
+    wj = torch.empty(m, dtype=torch.float32, device=dev)
+    work = torch.empty(n, dtype=torch.float32, device=dev)
+    wi = torch.empty(m, dtype=torch.float32, device=dev)
+    ws = torch.empty(m, dtype=torch.float32, device=dev)
+
+    def ec():
+        api.ec_scan(dg, dest, wj)
+        api.ec_unweighted(dg, dest, wj)
+
+    def vc():
+        api.rowsum(dg, work, v=W)
+        api.fill(wi, 0.0)
+        api.intersect(dg, work, wi, ws, v=W)
+        api.jw(wi, ws, wj)
+
+    res = {}
+    for name, fn in (("ec_scan_plus_ec_unweighted_ms", ec), ("rowsum_fill_is_jw_ms", vc)):
+        fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            ts.append((time.perf_counter() - t0) * 1e3)
+        res[name] = sorted(ts)[1]
+    ec()
+    fused = api.jaccard(dg, torch.empty_like(out), edge_centric=True)
+    torch.cuda.synchronize()
+    res["ec_kernel_level_equals_fused"] = bool(torch.equal(wj.view(torch.int32), fused.view(torch.int32)))
+    del dest, wj, work, wi, ws, fused
+    return res
+
+
 def run_batched(args):
     """Batched workload: 4096 RMAT graphs (n log-uniform in [2^8, 2^16], edge
     factor uniform in [4, 32], half weighted) in one captured HIP graph vs one
@@ -542,6 +586,7 @@ def main():
                 api.jaccard(dg, torch.empty_like(out), weights=wts, **kw)
             torch.cuda.synchronize()
             extras[f"{name}_ms"] = (time.perf_counter() - t1) * 1e3 / 3
+        extras["kernel_level"] = run_kernel_level(dg, W if weighted else None, out)
 
     if args.batched:
         extras["batched"] = run_batched(args)

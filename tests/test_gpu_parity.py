@@ -213,6 +213,39 @@ def test_pairs(cuda):
     assert np.isnan(api.jaccard_list(dg, bad, ok).cpu().numpy()).all()
 
 
+def test_length_class_boundaries(cuda):
+    """Owned rows on both sides of every streaming class limit (8, 16, 32, 64
+    and 512 entries per chunk) under an owner whose keys fit one table chunk
+    and under one that needs several (30,000 entries): EC and VC bit-exact
+    unweighted, weighted within tolerance."""
+    rng = np.random.default_rng(11)
+    n = 60000
+    lens = [8, 9, 16, 17, 32, 33, 64, 65, 511, 512, 513, 514, 1025, 1026, 1537, 1540, 2000]
+    L = len(lens)
+    u1, u2 = 0, L + 1          # owners: 2,500 and 30,000 entries
+    src, dst = [], []
+    for i, d in enumerate(lens):
+        x = 1 + i
+        nb = rng.choice(np.arange(L + 2, n), size=d - 2, replace=False)
+        src += [x] * d
+        dst += [u1, u2] + nb.tolist()
+    for u, k in ((u1, 2500), (u2, 30000)):
+        nb = rng.choice(np.arange(L + 2, n), size=k, replace=False)
+        src += [u] * k
+        dst += nb.tolist()
+    g = graphs.from_edges(n, np.array(src), np.array(dst))
+    deg = np.diff(g.offsets)
+    assert all(deg[1 + i] == d for i, d in enumerate(lens))
+    print(f"[gpu] length classes n={g.n} m={g.m} max_deg={deg.max()}", flush=True)
+    ref = oracle.ec_unweighted(g)
+    assert_bits_equal(run(g, cuda, ec=True), ref, "class limits EC")
+    assert_bits_equal(run(g, cuda, ec=False), ref, "class limits VC")
+    v = (rng.integers(1, 1 << 24, g.n) / float(1 << 24)).astype(np.float32)
+    refw = oracle.weighted_f64(g, v.astype(np.float64))
+    for ec in (True, False):
+        assert rel_err(run(g, cuda, v, ec=ec), refw).max() <= WEIGHTED_RTOL
+
+
 def test_row_sums_at_length_boundaries(cuda):
     """Row sums route rows by length (4 lanes up to 64 entries, 16 per step;
     a whole wave up to 2048; a workgroup beyond); rows on both sides of each

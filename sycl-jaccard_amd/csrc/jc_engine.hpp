@@ -1006,9 +1006,10 @@ __device__ __forceinline__ void unit(const EngineArgs& a, const Chunk<W>& cx, bo
 
 // Length classes of a short row (entries streamed per chunk):
 //   0: <= 8     G 8,  U 1      3: <= 64   G 16, U 4
-//   1: <= 16    G 16, U 1      4: <= 512  G 32, U 4
-//   2: <= 32    G 16, U 2      5: > 512   G 64, U 4 (weighted: 3;
+//   1: <= 16    G 8,  U 2      4: <= 512  G 32, U 4
+//   2: <= 32    G 8,  U 4      5: > 512   G 64, U 4 (weighted: 3;
 //                                          unweighted bitmap rows: 6)
+// (classes 1 and 2 in the one-wave-per-row kernel: G 16, U 1 and 2)
 // (Class 4 up to 512 entries rather than 256: two rows per wave in flight
 // instead of one, measured 58.7 -> 57.2 ms on RMAT-22 weighted, 3.45 -> 3.35
 // ms on Chung-Lu 2^20, 194 -> 187 ms for the batched step; 384 to 768 were
@@ -1041,16 +1042,21 @@ constexpr int kClasses = 6;
 __host__ __device__ __forceinline__ int len_class(int len) {
   return len <= 8 ? 0 : len <= 16 ? 1 : len <= 32 ? 2 : len <= 64 ? 3 : len <= JC_C4_MAX ? 4 : 5;
 }
-__host__ __device__ __forceinline__ constexpr int class_g(int c) {
-  return c == 0 ? 8 : c <= 3 ? 16 : c == 4 ? 32 : 64;
+// Lanes per group of a class. The workgroup kernel streams rows of up to 32
+// entries with 8 lanes (8 rows per wave; 16 lanes measured 0.8% slower on
+// RMAT-22 weighted and 3% on Chung-Lu 2^20); the one-wave-per-row kernel
+// (small) keeps 16 lanes for classes 1-2, where 8 measured 4% slower.
+__host__ __device__ __forceinline__ constexpr int class_g(int c, bool small = false) {
+  return (c == 0 || (!small && c <= 2)) ? 8 : c <= 3 ? 16 : c == 4 ? 32 : 64;
 }
 
 template <int MODE, bool W, bool MULTI, int C, int TK = kTkHash, bool SMALL = false>
 __device__ __forceinline__ void unit_class(const EngineArgs& a, const Chunk<W>& cx, bool valid, int32_t x,
                                            int ox, int dx, int j, int item, bool fast) {
-  constexpr int G = class_g(C);
-  constexpr int U = C <= 1 ? 1
-                  : C == 2 ? 2
+  constexpr int G = class_g(C, SMALL);
+  constexpr int U = C == 0 ? 1
+                  : C == 1 ? (G == 8 ? 2 : 1)
+                  : C == 2 ? (G == 8 ? 4 : 2)
                   : C == 3 ? 4
                   : C == 4 ? (TK == kTkBitmap ? JC_U4BM : JC_U4)
                            : (W ? (TK == kTkBitmap ? JC_U5WBM : JC_U5W) : (TK == kTkBitmap ? JC_U5BM : JC_U5));
@@ -1076,7 +1082,7 @@ __device__ __forceinline__ void unit_class(const EngineArgs& a, const Chunk<W>& 
 template <int MODE, bool W, int C>
 __device__ __forceinline__ void run_class_regs(const EngineArgs& a, const Chunk<W>& cx, uint64_t mask,
                                                int32_t x, int ox, int dx, int j, bool fast) {
-  constexpr int G = class_g(C);
+  constexpr int G = class_g(C, true);
   constexpr int NG = kWave / G;
   const int g = lane_id() / G;
   while (mask) {

@@ -204,23 +204,31 @@ struct Table {
   }
 };
 
-// Average keys per bucket (of kBucketKeys key slots) at full chunk size:
-// 3 for key-only tables, 2 for weighted ones (whose LDS holds weights too;
-// denser buckets spill more and cost more there than fewer chunks save).
+// Average keys per bucket (of kBucketKeys key slots) at which the workgroup
+// kernel cuts a long row into chunks: 3, weighted or not (weighted rows were
+// cut at 2; fewer, fuller chunks measured faster, as chunk restarts cost more
+// than the overflow walks they save: RMAT-22 weighted on 32-bit tables 68.5
+// -> 65.8 ms). The one-wave-per-row kernel sizes its private tables for at
+// most kLambdaSmall keys per bucket (2 weighted, where 3 measured slower).
 #ifndef JC_LAMBDA_U
 #define JC_LAMBDA_U 3
 #endif
 #ifndef JC_LAMBDA_W
-#define JC_LAMBDA_W 2
+#define JC_LAMBDA_W 3
+#endif
+#ifndef JC_LAMBDA_SMALL_W
+#define JC_LAMBDA_SMALL_W 2
 #endif
 template <bool W>
 constexpr int kLambda = W ? JC_LAMBDA_W : JC_LAMBDA_U;
+template <bool W>
+constexpr int kLambdaSmall = W ? JC_LAMBDA_SMALL_W : JC_LAMBDA_U;
 
-// log2 of the bucket count for a chunk of `len` keys: at most kLambda keys
-// per bucket on average, at least 4 buckets.
+// log2 of the bucket count for a chunk of `len` keys in the one-wave-per-row
+// kernel: at most kLambdaSmall keys per bucket on average, at least 4 buckets.
 template <bool W>
 __device__ __forceinline__ int table_log_buckets(int len) {
-  return max(ceil_log2((len + kLambda<W> - 1) / kLambda<W>), 2);
+  return max(ceil_log2((len + kLambdaSmall<W> - 1) / kLambdaSmall<W>), 2);
 }
 
 // The same for the workgroup kernel, whose table region always holds max_lb

@@ -1015,8 +1015,8 @@ __device__ __forceinline__ void unit(const EngineArgs& a, const Chunk<W>& cx, bo
 // Length classes of a short row (entries streamed per chunk):
 //   0: <= 8     G 8,  U 1      3: <= 64   G 16, U 4
 //   1: <= 16    G 8,  U 2      4: <= 512  G 32, U 4
-//   2: <= 32    G 8,  U 4      5: > 512   G 64, U 4 (weighted: 3;
-//                                          unweighted bitmap rows: 6)
+//   2: <= 32    G 8,  U 4      5: > 512   G 64, U 4 (weighted bitmap
+//                                          rows: 3; unweighted bitmap rows: 6)
 // (classes 1 and 2 in the one-wave-per-row kernel: G 16, U 1 and 2)
 // (Class 4 up to 512 entries rather than 256: two rows per wave in flight
 // instead of one, measured 58.7 -> 57.2 ms on RMAT-22 weighted, 3.45 -> 3.35
@@ -1029,7 +1029,7 @@ __device__ __forceinline__ void unit(const EngineArgs& a, const Chunk<W>& cx, bo
 #define JC_U5 4
 #endif
 #ifndef JC_U5W
-#define JC_U5W 3  // weighted probes hold more registers: three loads per lane measured best
+#define JC_U5W 4  // four loads per lane: best once rows of up to 512 entries went to class 4 (3 before)
 #endif
 // the same for rows probed against a bitmap table; the cheaper unweighted
 // bitmap probe keeps more loads in flight (six measured best on the batched
@@ -1041,7 +1041,7 @@ __device__ __forceinline__ void unit(const EngineArgs& a, const Chunk<W>& cx, bo
 #define JC_U5BM 6
 #endif
 #ifndef JC_U5WBM
-#define JC_U5WBM JC_U5W
+#define JC_U5WBM 3  // weighted bitmap rows: three loads per lane
 #endif
 #ifndef JC_C4_MAX
 #define JC_C4_MAX 512
